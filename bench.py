@@ -1,0 +1,180 @@
+"""Benchmark: primary rays/s at 4K on one MI355X, full pass sequence.
+
+Workload (one "step" = one frame): the 16x4x16-chunk seeded world with a
+5-level clipmap, frame f of the 120-frame seeded camera flight, rendered at
+3840x2160 (render scale 1.0) through primary -> local TAA -> godray ->
+composite -> composite TAA.  Inputs are resident in HBM before timing starts;
+each step writes the 464-byte camera and launches the frame.
+
+Prints one JSON line.  value = primary rays per second of the whole frame
+(rays per frame / wall time per frame).  Also reports the primary kernel's
+own rate, per-pass milliseconds, p50/p99 per-frame latency, the primary
+kernel's roofline position (algorithmic bytes from the counting variant of
+the same kernel / its event-timed duration, against 8 TB/s) and a CPU
+baseline: the oracle (C++ restatement of the reference passes) on a bounded
+sample of the same frame on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "voxel-engine-v0_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--res", default="3840x2160")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline)")
+    ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU sample (0: auto)")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(the path shards by request across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import voxhip as vx
+    from voxhip import scenes
+
+    W, H = (int(v) for v in a.res.lower().split("x"))
+    t0 = time.time()
+    sc = scenes.c1(resolution=(W, H))
+    hw = vx.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
+    clip, heights = scenes.scene_clipmap(sc, 0)
+    ctx = vx.Context(node_capacity=len(hw.nodes), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
+                     clip_levels=5, clip_res=256)
+    ctx.upload_world(hw)
+    ctx.write_clipmap(clip, heights)
+    view = vx.View(ctx, (W, H), (W, H))
+    setup_s = time.time() - t0
+
+    cams = []
+    prev = None
+    for f in range(a.warmup + a.steps):
+        cam = scenes.frame_camera(sc, hw, f, prev_view_proj=prev, jitter=True)
+        prev = np.array(cam.view_proj, np.float32)
+        cams.append(cam)
+
+    for f in range(a.warmup):
+        view.write_camera(cams[f])
+        view.render(vx.PASS_FRAME)
+    ctx.sync()
+
+    # Timed region: K frames back to back.
+    lat = []
+    pass_ms = []
+    t_start = time.perf_counter()
+    for k in range(a.steps):
+        f = a.warmup + k
+        t_f = time.perf_counter()
+        view.write_camera(cams[f])
+        view.render(vx.PASS_FRAME)
+        ms = view.timings()  # waits for the frame: per-frame latency is arrival -> result ready
+        lat.append((time.perf_counter() - t_f) * 1e3)
+        pass_ms.append(ms)
+    ctx.sync()
+    total_s = time.perf_counter() - t_start
+
+    rays = W * H
+    ms_per_step = total_s * 1e3 / a.steps
+    pass_ms = np.array(pass_ms)
+    prim_ms = float(np.mean(pass_ms[:, 0]))
+    value = rays * a.steps / total_s
+
+    roofline = None
+    counts = None
+    if not a.no_count:
+        # Same kernel, counting variant, over the first three timed frames.
+        view.reset()
+        tot = {n: 0 for n in vx.COUNT_NAMES}
+        ncount = min(3, a.steps)
+        view.counters(reset=True)
+        for k in range(ncount):
+            view.write_camera(cams[a.warmup + k])
+            view.render(vx.PASS_PRIMARY | vx.PASS_COUNT | vx.PASS_END_FRAME)
+            c = view.counters(reset=True)
+            for n in tot:
+                tot[n] += c[n]
+        counts = {n: v / ncount for n, v in tot.items()}
+        bytes_per_launch = vx.algorithmic_bytes(counts)
+        achieved = bytes_per_launch / (prim_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "kernel": "k_primary", "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                    "algorithmic_bytes_per_ray": round(bytes_per_launch / rays, 1)}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(sc, hw, clip, heights, cams[a.warmup], W, H, a.cpu_rows)
+
+    out = {
+        "metric": "primary rays/s at 4K + p50/p99 per-view latency, served batched vs plain (1 GPU)",
+        "value": round(value, 1),
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seed-42 world, seeded camera flight)",
+        "config": {"workload": f"plain one-view-per-launch full pipeline at {W}x{H}, 16x4x16-chunk world, "
+                               f"5-level clipmap, camera flight frames {a.warmup}..{a.warmup + a.steps - 1}",
+                   "resolution": [W, H], "chunks": hw.chunk_count, "nodes": int(len(hw.nodes)),
+                   "path": "plain"},
+        "primary_kernel_rays_per_s": round(rays / (prim_ms * 1e-3), 1),
+        "pass_ms_mean": {n: round(float(v), 4) for n, v in
+                         zip(("primary", "local_taa", "godray", "composite", "composite_taa"), pass_ms.mean(axis=0))},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "setup_s": round(setup_s, 2),
+    }
+    if counts is not None:
+        out["primary_counts_per_launch"] = {k: int(v) for k, v in counts.items()}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(sc, hw, clip, heights, cam, W, H, rows):
+    """The oracle's main_primary on a band of tile rows through the middle of
+    the same 4K frame, timed on the host cores (bounded to ~10-30 s)."""
+    sys.path.insert(0, str(REPO))
+    from oracle import oracle as O
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    tx = (W + 7) // 8
+    ty = (H + 7) // 8
+    if rows <= 0:
+        rows = max(1, min(ty, int(8 * threads)))  # tile rows in the sample
+    r0 = max(0, ty // 2 - rows // 2)
+    w = O.World.from_host(hw, clip, heights)
+    fr = O.Frame(W, H)
+    t = time.perf_counter()
+    O.primary(w, cam, fr, tiles=(r0 * tx, (r0 + rows) * tx), threads=threads)
+    dt = time.perf_counter() - t
+    n = rows * 8 * W
+    return {"value": round(n / dt, 1), "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"main_primary over tile rows {r0}..{r0 + rows - 1} ({n} rays) of frame 0 at {W}x{H}, "
+                      f"{dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,364 @@
+/*
+ * voxhip.h — C ABI of libvoxhip, the MI355X (gfx950) implementation of the
+ * voxel-engine-v0 primary / local-TAA / godray / composite passes.
+ *
+ * Two groups of entry points:
+ *   1. The renderer boundary.  It replaces the reference's `render::Renderer`
+ *      (src/render/state/mod.rs:34-712) and the WGSL binding table
+ *      (src/shaders/common.wgsl:403-409, src/shaders/ray_main.wgsl:20-59).
+ *      Inputs are the reference's GPU byte formats, unchanged
+ *      (src/render/gpu_types.rs:15-138).
+ *   2. The CPU producers of those inputs.  They replace the chunk builder
+ *      (src/svo/builder.rs:846-986), the chunk meta/grid packing
+ *      (src/streaming/manager/slots.rs:231-251, grid.rs:15-72), the camera
+ *      packing (src/app/app.rs:411-511) and the clipmap fill
+ *      (src/clipmap.rs:96-122,182-577).
+ *
+ * Conventions: every function returns an int status (VX_OK = 0, negative on
+ * error) and never unwinds across the ABI.  vx_last_error() returns a
+ * thread-local message for the last failing call on that thread.  Host
+ * pointers are read before the call returns (the caller keeps ownership).
+ * A context is externally synchronised: calls on one context (and its views)
+ * must not overlap.
+ */
+#ifndef VOXHIP_H
+#define VOXHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                             */
+/* ------------------------------------------------------------------------ */
+#define VX_OK            0
+#define VX_E_INVALID    -1  /* bad argument (null pointer, size, enum)       */
+#define VX_E_CAPACITY   -2  /* write past a capacity fixed at vx_create      */
+#define VX_E_DEVICE     -3  /* HIP runtime error                             */
+#define VX_E_NOMEM      -4  /* device or host allocation failed              */
+#define VX_E_NODEVICE   -5  /* no usable GPU                                 */
+#define VX_E_BUSY       -6  /* serving: request rejected by admission        */
+#define VX_E_AGAIN      -7  /* serving: nothing finished yet                 */
+
+#define VX_INVALID_U32  0xFFFFFFFFu
+
+/* Materials (src/world/materials.rs:1-7, common.wgsl:37-43). */
+#define VX_MAT_AIR   0u
+#define VX_MAT_GRASS 1u
+#define VX_MAT_DIRT  2u
+#define VX_MAT_STONE 3u
+#define VX_MAT_WOOD  4u
+#define VX_MAT_LEAF  5u
+#define VX_MAT_LIGHT 6u
+
+#define VX_CHUNK_SIZE        64u     /* app/config.rs:5                      */
+#define VX_MACRO_WORDS       16u     /* common.wgsl:417                      */
+#define VX_COLINFO_WORDS     2048u   /* svo/builder.rs:747                   */
+#define VX_CLIP_LEVELS_MAX   16u     /* common.wgsl:241, app/config.rs:40    */
+#define VX_MAX_CHUNK_NODES   299593u /* 64^3+32^3+16^3+8^3+4^3+2^3+1         */
+
+/* ------------------------------------------------------------------------ */
+/* Byte formats (identical to src/render/gpu_types.rs)                      */
+/* ------------------------------------------------------------------------ */
+
+/* NodeGpu, gpu_types.rs:15-20.  child_base == 0xFFFFFFFF marks a leaf.
+ * key = x | y<<6 | z<<12 | level<<18 (svo/builder_svo.rs:22-35). */
+typedef struct vx_node {
+    uint32_t child_base;
+    uint32_t child_mask;
+    uint32_t material;
+    uint32_t key;
+} vx_node;
+
+/* NodeRopesGpu, gpu_types.rs:24-33.  Chunk-local neighbour indices. */
+typedef struct vx_node_ropes {
+    uint32_t px, nx, py, ny, pz, nz;
+    uint32_t pad0, pad1;
+} vx_node_ropes;
+
+/* ChunkMetaGpu, gpu_types.rs:38-48. */
+typedef struct vx_chunk_meta {
+    int32_t  origin[4];      /* chunk origin in voxels */
+    uint32_t node_base;      /* arena offset (elements) of this chunk's nodes/ropes */
+    uint32_t node_count;
+    uint32_t macro_base;     /* = slot * 16  */
+    uint32_t colinfo_base;   /* = slot * 2048 */
+    uint32_t macro_empty;    /* 1 iff all 16 macro words are zero */
+    uint32_t pad0, pad1, pad2;
+} vx_chunk_meta;
+
+/* CameraGpu, gpu_types.rs:52-86 (464 bytes, column-major matrices). */
+typedef struct vx_camera {
+    float    view_inv[16];
+    float    proj_inv[16];
+    float    view_proj[16];
+    float    prev_view_proj[16];
+    float    cam_pos[4];
+    float    ray00[4];
+    float    ray_dx[4];
+    float    ray_dy[4];
+    uint32_t chunk_size;
+    uint32_t chunk_count;
+    uint32_t max_steps;
+    uint32_t frame_index;
+    float    voxel_params[4];      /* [voxel_size_m, time_s, wind_strength, fog_density] */
+    int32_t  grid_origin_chunk[4];
+    uint32_t grid_dims[4];
+    uint32_t render_present_px[4]; /* render w, render h, present w, present h */
+    uint32_t profile_flags;
+    uint32_t pad_profile[3];
+    uint32_t pad0[4];
+    uint32_t pad1[4];
+    uint32_t pad2[4];
+} vx_camera;
+
+/* ClipmapGpu, gpu_types.rs:105-113 (528 bytes). */
+typedef struct vx_clipmap {
+    uint32_t levels;
+    uint32_t res;
+    float    base_cell_m;
+    float    pad0;
+    float    level[16][4];   /* origin_x_m, origin_z_m, cell_m, 0 */
+    uint32_t offset[16][4];  /* toroidal offsets off_x, off_z, 0, 0 */
+} vx_clipmap;
+
+/* Per-pixel hit identity (not present in the reference, whose HitGeom carries
+ * no node index: ray/chunk_trace.wgsl:15-32).  Written when requested. */
+typedef struct vx_hit {
+    uint32_t slot;     /* chunk slot of the hit, VX_INVALID_U32 if none          */
+    uint32_t node;     /* chunk-local leaf node index, VX_INVALID_U32 for grass  */
+    int32_t  voxel[3]; /* world voxel of the hit leaf's min corner / grass cell  */
+    uint32_t mat;      /* material, VX_MAT_AIR for miss                          */
+    float    t;        /* hit distance in metres (BIG for miss)                  */
+    uint32_t kind;     /* 0 sky, 1 voxel, 2 heightfield                           */
+} vx_hit;
+
+/* ------------------------------------------------------------------------ */
+/* 1. Renderer boundary                                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vx_ctx  vx_ctx;
+typedef struct vx_view vx_view;
+
+/* Capacities (render/state/buffers.rs:79-127).  Zero means the reference's
+ * default: 67,108,864 nodes, 12,996 chunks/grid cells, 16 clip levels x 256^2. */
+typedef struct vx_caps {
+    uint64_t node_capacity;     /* elements of the node and rope arenas        */
+    uint32_t chunk_capacity;    /* chunk meta slots (macro = x16, colinfo = x2048) */
+    uint32_t grid_capacity;     /* chunk grid cells                             */
+    uint32_t clip_levels;       /* clip height array layers                     */
+    uint32_t clip_res;          /* clip height array resolution                 */
+    int32_t  device;            /* HIP device ordinal                           */
+    uint32_t flags;             /* VX_CTX_* */
+} vx_caps;
+
+/* By default the godray pass reads macro occupancy through the rope buffer, as
+ * the reference does: its scene bind group puts the ropes at the binding the
+ * shader declares for macro_occ (render/state/bindgroups.rs:142-151 vs
+ * common.wgsl:407-409).  VX_CTX_GODRAY_MACRO_FIX makes it read the macro words. */
+#define VX_CTX_GODRAY_MACRO_FIX 1u
+
+/* Renderer::new (render/state/mod.rs:147-268): allocates the resident world. */
+int  vx_create(const vx_caps* caps, vx_ctx** out);
+void vx_destroy(vx_ctx* ctx);
+
+/* Arenas addressed by vx_upload_regions (render/state/mod.rs:514-582). */
+#define VX_ARENA_NODES   0
+#define VX_ARENA_ROPES   1
+#define VX_ARENA_META    2
+#define VX_ARENA_MACRO   3
+#define VX_ARENA_COLINFO 4
+
+typedef struct vx_region {
+    uint64_t    offset;    /* element offset in the arena                      */
+    uint64_t    count;     /* elements                                         */
+    const void* data;      /* host pointer, count * element size bytes         */
+} vx_region;
+
+/* Renderer::apply_chunk_uploads (render/state/mod.rs:514-582).  Regions past
+ * the capacity fail with VX_E_CAPACITY (the reference skips them silently). */
+int vx_upload_regions(vx_ctx* ctx, int arena, const vx_region* regions, uint32_t n);
+
+/* Renderer::write_chunk_grid (render/state/mod.rs:310-317).  The grid origin and
+ * dims travel in vx_camera, as in the reference; here only the cell values. */
+int vx_write_grid(vx_ctx* ctx, const uint32_t* grid, uint32_t cells);
+
+/* Renderer::write_clipmap_updates (render/state/mod.rs:586-634).  Patches are
+ * rectangles of one level in storage (toroidal) texel coordinates. */
+typedef struct vx_clip_patch {
+    uint32_t level, x, y, w, h;
+    const float* data;     /* w*h floats, row-major */
+} vx_clip_patch;
+int vx_write_clipmap(vx_ctx* ctx, const vx_clipmap* params, const vx_clip_patch* patches, uint32_t n);
+
+/* A view: one camera stream with its own history (the reference's texture set,
+ * render/state/textures.rs:122-285, and ping-pong, bindgroups.rs:305-459).
+ * render = internal resolution of primary/local_taa/godray; present = output. */
+int  vx_view_create(vx_ctx* ctx, uint32_t present_w, uint32_t present_h,
+                    uint32_t render_w, uint32_t render_h, vx_view** out);
+void vx_view_destroy(vx_view* view);
+/* Renderer::resize_output (render/state/mod.rs:297-308): reallocates and clears history. */
+int  vx_view_resize(vx_view* view, uint32_t present_w, uint32_t present_h,
+                    uint32_t render_w, uint32_t render_h);
+/* Clears all history textures to zero (a fresh Renderer). */
+int  vx_view_reset(vx_view* view);
+
+/* Renderer::write_camera (render/state/mod.rs:319-322). */
+int vx_write_camera(vx_view* view, const vx_camera* cam);
+
+/* Renderer::encode_compute (render/state/mod.rs:329-446).  pass_mask selects
+ * the passes run this call, in reference order, against the current history
+ * slot; VX_PASS_END_FRAME then flips the ping-pong (render/state/mod.rs:445),
+ * so a frame may be split over several calls.  stream: a hipStream_t or NULL
+ * (the context's stream).  Launches are stream-ordered; the call does not block. */
+#define VX_PASS_PRIMARY        0x01u
+#define VX_PASS_LOCAL_TAA      0x02u
+#define VX_PASS_GODRAY         0x04u
+#define VX_PASS_COMPOSITE      0x08u
+#define VX_PASS_COMPOSITE_TAA  0x10u
+#define VX_PASS_END_FRAME      0x80u   /* flip the history ping-pong after the passes */
+#define VX_PASS_FRAME          0x9Fu   /* all five passes + end of frame */
+#define VX_PASS_WRITE_HITS     0x100u  /* also write the vx_hit buffer in primary */
+#define VX_PASS_COUNT          0x200u  /* run the load-counting variant of primary/godray */
+int vx_render(vx_view* view, uint32_t pass_mask, void* stream);
+
+/* Output selectors for vx_read_output / vx_output_device_ptr.  Linear row-major
+ * arrays; RGBA32F = 16 B/px, R32F = 4 B/px.  History selectors name the copy the
+ * NEXT frame will read (i.e. the one this frame wrote). */
+#define VX_OUT_COLOR          0  /* render res RGBA32F (primary colour)        */
+#define VX_OUT_DEPTH          1  /* render res R32F                            */
+#define VX_OUT_LOCAL          2  /* render res RGBA32F                         */
+#define VX_OUT_PRIMARY_HIST   3  /* render res RGBA32F                         */
+#define VX_OUT_SHADOW_HIST    4  /* render res R32F (dense, no row padding)    */
+#define VX_OUT_LOCAL_HIST     5  /* render res RGBA32F                         */
+#define VX_OUT_GODRAY         6  /* render res RGBA32F                         */
+#define VX_OUT_PRE_TAA        7  /* present res RGBA32F (composite output)     */
+#define VX_OUT_OUTPUT         8  /* present res RGBA32F (composite TAA output) */
+#define VX_OUT_HITS           9  /* render res vx_hit                          */
+/* Synchronous device-to-host copy of one output (waits for the view's stream). */
+int vx_read_output(vx_view* view, int which, void* dst, size_t bytes);
+/* Device pointer of one output, for zero-copy consumers on the same device. */
+int vx_output_device_ptr(vx_view* view, int which, void** dptr, size_t* bytes);
+/* Host-to-device write of a history texture (tests seed history with it). */
+int vx_write_history(vx_view* view, int which, const void* src, size_t bytes);
+
+/* Algorithmic load counters accumulated by VX_PASS_COUNT runs since the last
+ * reset: elements loaded from [grid, meta, macro, node, rope, colinfo, clip]
+ * and texture bytes [read, written]. */
+int vx_view_counters(vx_view* view, uint64_t out[9], int reset);
+
+/* Per-pass GPU milliseconds of the last vx_render that ran them
+ * (render/state/mod.rs:649-709): [primary, local_taa, godray, composite, composite_taa].
+ * Waits for the view's stream. */
+int vx_view_timings(vx_view* view, float ms[5]);
+
+/* Ray queries (no reference counterpart beyond app/app.rs:847-961's CPU pick):
+ * first voxel hit of each ray against the resident world, hit identity only.
+ * Rays and hits are DEVICE pointers; the call is stream-ordered. */
+typedef struct vx_ray {
+    float ox, oy, oz, tmax;   /* origin (metres), max distance (metres) */
+    float dx, dy, dz, pad;    /* direction (unit) */
+} vx_ray;
+int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev,
+                  uint32_t n, vx_hit* hits_dev, void* stream);
+
+/* Blocks until all work queued on the context's stream is done. */
+int vx_sync(vx_ctx* ctx);
+
+const char* vx_last_error(void);
+/* Library build string: compiler, target, flags. */
+const char* vx_version(void);
+/* 1 if a usable GPU is present, 0 otherwise (no error). */
+int vx_device_count(void);
+
+/* ------------------------------------------------------------------------ */
+/* 2. CPU producers of the inputs                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Chunk builder, svo/builder.rs:846-986 stages col tops -> colinfo -> prefix ->
+ * macro bits -> min/max mip -> bottom-up SVO -> ropes.  material is the y-slab
+ * layout idx = ly*4096 + lz*64 + lx (svo/builder.rs:599-658); ground is the
+ * per-column ground height in world voxels (idx = lz*64 + lx), used only by the
+ * stone-collapse rule (svo/builder_svo.rs:144-181).  nodes/ropes must hold
+ * node_cap elements (VX_MAX_CHUNK_NODES always suffices). */
+int vx_build_chunk(const uint8_t* material, const int32_t* ground, int32_t chunk_oy,
+                   int32_t dirt_depth, vx_node* nodes, vx_node_ropes* ropes,
+                   uint32_t node_cap, uint32_t* node_count,
+                   uint32_t macro_words[16], uint32_t colinfo_words[2048]);
+
+/* The seeded synthetic world: 2-octave value-noise heightfield, stone/dirt/grass
+ * layers and trees with leaf blobs.  Its definition is this library's (the
+ * reference world generator, world/generator.rs, is not used). */
+typedef struct vx_world_params {
+    uint32_t seed;          /* 42 */
+    int32_t  base_y;        /* 96 voxels */
+    int32_t  amplitude;     /* 48 voxels */
+    int32_t  dirt_depth;    /* 4: stone below ground-4, then dirt, grass top */
+    float    tree_density;  /* 0.01 of columns */
+    int32_t  trunk_min;     /* 5 */
+    int32_t  trunk_max;     /* 12 */
+    int32_t  leaf_radius;   /* 2 => 5^3 leaf blob */
+    int32_t  wavelength0;   /* 64 voxels (octave 0); octave 1 is half */
+} vx_world_params;
+
+void    vx_world_default(vx_world_params* p);
+int32_t vx_world_ground_height(const vx_world_params* p, int32_t wx, int32_t wz);
+/* Fills one chunk's material (y-slab layout) and ground (lz*64+lx) arrays. */
+int vx_world_fill_chunk(const vx_world_params* p, int32_t cx, int32_t cy, int32_t cz,
+                        uint8_t* material, int32_t* ground);
+
+/* A whole resident world built on the host: every chunk of the grid built with
+ * vx_build_chunk (threads: 0 = hardware concurrency), packed into arenas in
+ * grid order (x fastest, then y, then z), one slot per non-empty chunk,
+ * node_base assigned contiguously (try_make_uploading, slots.rs:163-275). */
+typedef struct vx_host_world vx_host_world;
+int  vx_host_world_build(const vx_world_params* p, const int32_t grid_origin[3],
+                         const uint32_t grid_dims[3], uint32_t threads, vx_host_world** out);
+void vx_host_world_free(vx_host_world* w);
+typedef struct vx_host_world_view {
+    const vx_node*       nodes;    uint64_t node_count;
+    const vx_node_ropes* ropes;
+    const vx_chunk_meta* meta;     uint32_t chunk_count;
+    const uint32_t*      macro_words;   /* chunk_count * 16   */
+    const uint32_t*      colinfo_words; /* chunk_count * 2048 */
+    const uint32_t*      grid;     uint32_t grid_cells;
+    int32_t  grid_origin[3];
+    uint32_t grid_dims[3];
+} vx_host_world_view;
+int vx_host_world_get(const vx_host_world* w, vx_host_world_view* out);
+/* Applies single-voxel edits (world voxel coords, material) and rebuilds the
+ * touched chunks in place on the host; returns the rebuilt slots.  Used by the
+ * serving edit stream. */
+int vx_host_world_edit(vx_host_world* w, const int32_t* xyz, const uint8_t* mats,
+                       uint32_t n, uint32_t* rebuilt_slots, uint32_t* n_rebuilt);
+
+/* Camera pose -> CameraGpu, app/app.rs:411-511 and app/camera.rs:95-111
+ * (glam look_at_rh / perspective_rh, Halton(2,3) jitter of frame_index+1). */
+typedef struct vx_pose {
+    float pos[3];
+    float yaw, pitch;       /* radians, camera.rs:96-100 */
+    float fovy;             /* radians */
+    float z_near, z_far;
+} vx_pose;
+#define VX_CAM_JITTER 1u
+int vx_camera_build(const vx_pose* pose, uint32_t render_w, uint32_t render_h,
+                    uint32_t present_w, uint32_t present_h, uint32_t frame_index,
+                    float time_s, float wind_strength, float fog_density,
+                    const int32_t grid_origin[3], const uint32_t grid_dims[3],
+                    uint32_t chunk_count, const float* prev_view_proj /* 16 or NULL */,
+                    uint32_t flags, vx_camera* out);
+
+/* Full clipmap for a camera position, clipmap.rs:62-122 (snapped origins,
+ * texel-centre heights) with toroidal offsets = origin cell mod res.
+ * heights: levels*res*res floats, layer-major, row-major (storage coordinates). */
+int vx_clipmap_build(const vx_world_params* p, float cam_x, float cam_z, uint32_t levels,
+                     uint32_t res, float base_cell_m, vx_clipmap* out, float* heights);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOXHIP_H */

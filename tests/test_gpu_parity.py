@@ -1,0 +1,135 @@
+"""GPU parity: the HIP passes against the oracle on the same seeded inputs.
+
+hit identity bit-exact, depth within 1e-5 relative, colour
+within 1e-3 absolute (parity.py).  All runs go through the C ABI.
+"""
+from __future__ import annotations
+
+import json
+import math
+
+import numpy as np
+import pytest
+
+from conftest import host_world
+from parity import COLOR_ABS_TOL, DEPTH_REL_TOL, make_context, summary
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle():
+    from oracle import oracle as O
+
+    return O
+
+
+def _report(name, s):
+    print(f"[parity] {name}: {json.dumps(s)}", flush=True)
+
+
+def _assert_parity(s):
+    assert s["identity_mismatches"] == 0, s
+    assert s["depth_max_rel"] <= DEPTH_REL_TOL, s
+    assert s["color_max_abs"] <= COLOR_ABS_TOL, s
+
+
+@pytest.mark.parametrize("wind", [0.0, 2.0])
+def test_primary_c0_first_frame(gpu, wind):
+    """Config 0: plain primary pass, 640x360, fresh history, fixed seeded pose."""
+    O = _oracle()
+    from voxhip import scenes
+
+    sc, hw = host_world("c0")
+    clip, heights = scenes.scene_clipmap(sc)
+    cam = scenes.frame_camera(sc, hw, 0, wind_strength=wind)
+    ctx = make_context(gpu, hw)
+    ctx.write_clipmap(clip, heights)
+    view = gpu.View(ctx, sc.present, sc.render)
+    view.write_camera(cam)
+    view.render(gpu.PASS_PRIMARY | gpu.PASS_WRITE_HITS | gpu.PASS_END_FRAME)
+    hits = view.read(gpu.OUT_HITS)
+    depth = view.read(gpu.OUT_DEPTH)
+    color = view.read(gpu.OUT_COLOR)
+    local = view.read(gpu.OUT_LOCAL)
+    phist = view.read(gpu.OUT_PRIMARY_HIST)
+    shadow = view.read(gpu.OUT_SHADOW_HIST)
+
+    w = O.World.from_host(hw, clip, heights)
+    fr = O.Frame(*sc.render)
+    O.primary(w, cam, fr, gpu.HIT_DTYPE)
+    s = summary(hits, fr.hits, depth, fr.depth, color, fr.color)
+    s["local_max_abs"] = float(np.max(np.abs(local - fr.local)))
+    s["hist_bit_exact"] = bool(np.array_equal(phist.view(np.uint32), fr.primary_hist.view(np.uint32)))
+    s["shadow_max_abs"] = float(np.max(np.abs(shadow - fr.shadow)))
+    s["kinds"] = np.bincount(fr.hits["kind"].ravel(), minlength=3).tolist()
+    _report(f"c0 primary wind={wind}", s)
+    _assert_parity(s)
+    assert s["local_max_abs"] <= COLOR_ABS_TOL
+    assert s["shadow_max_abs"] <= COLOR_ABS_TOL
+    # Every ray class appears, so the comparison covers voxel, heightfield and sky paths.
+    assert min(s["kinds"]) > 0
+
+
+def test_full_pipeline_multiframe(gpu):
+    """Four full frames (primary, local TAA, godray, composite, composite TAA)
+    with a moving camera, so history reprojection, anchors and the 1/8 shadow
+    refresh feed back frame to frame on both sides."""
+    O = _oracle()
+    from voxhip import scenes
+
+    sc, hw = host_world("c0")
+    clip, heights = scenes.scene_clipmap(sc)
+    res = (316, 180)  # not multiples of 8: the last tile row and column are partial
+    ctx = make_context(gpu, hw)
+    ctx.write_clipmap(clip, heights)
+    view = gpu.View(ctx, res, res)
+    w = O.World.from_host(hw, clip, heights)
+    fr = O.Frame(*res)
+    pose0, _ = sc.poses[0]
+    prev = None
+    for f in range(4):
+        pose = gpu.make_pose((pose0.pos[0] + 0.05 * f, pose0.pos[1], pose0.pos[2] - 0.03 * f), pose0.yaw + 0.01 * f,
+                             pose0.pitch, 70.0)
+        cam = gpu.camera_build(pose, res, res, f, time_s=f / 60.0, wind_strength=2.0, grid_origin=sc.grid_origin,
+                               grid_dims=sc.grid_dims, chunk_count=hw.chunk_count, prev_view_proj=prev, jitter=True)
+        prev = np.array(cam.view_proj, np.float32)
+        view.write_camera(cam)
+        view.render(gpu.PASS_FRAME | gpu.PASS_WRITE_HITS)
+        O.frame(w, cam, fr, gpu.HIT_DTYPE)
+        s = summary(view.read(gpu.OUT_HITS), fr.hits, view.read(gpu.OUT_DEPTH), fr.depth, view.read(gpu.OUT_COLOR),
+                    fr.color)
+        out = view.read(gpu.OUT_OUTPUT)
+        s["output_max_abs"] = float(np.max(np.abs(out - fr.output)))
+        s["godray_max_abs"] = float(np.max(np.abs(view.read(gpu.OUT_GODRAY) - fr.godray)))
+        s["local_hist_max_abs"] = float(np.max(np.abs(view.read(gpu.OUT_LOCAL_HIST) - fr.local_hist)))
+        _report(f"pipeline frame {f}", s)
+        _assert_parity(s)
+        assert s["output_max_abs"] <= COLOR_ABS_TOL
+        assert s["godray_max_abs"] <= COLOR_ABS_TOL
+        assert s["local_hist_max_abs"] <= COLOR_ABS_TOL
+
+
+@pytest.mark.parametrize("fix", [False, True])
+def test_godray_macro_binding_modes(gpu, fix):
+    """Godray with the reference's rope-buffer macro binding (default) and the fixed binding."""
+    O = _oracle()
+    from voxhip import scenes
+
+    sc, hw = host_world("c0")
+    clip, heights = scenes.scene_clipmap(sc)
+    res = (256, 144)
+    ctx = make_context(gpu, hw, godray_macro_fix=fix)
+    ctx.write_clipmap(clip, heights)
+    view = gpu.View(ctx, res, res)
+    w = O.World.from_host(hw, clip, heights)
+    fr = O.Frame(*res)
+    pose, _ = sc.poses[0]
+    cam = gpu.camera_build(pose, res, res, 5, wind_strength=0.0, grid_origin=sc.grid_origin, grid_dims=sc.grid_dims,
+                           chunk_count=hw.chunk_count)
+    view.write_camera(cam)
+    view.render(gpu.PASS_PRIMARY | gpu.PASS_GODRAY | gpu.PASS_END_FRAME)
+    O.primary(w, cam, fr)
+    O.godray(w, cam, fr, macro_compat=not fix)
+    err = float(np.max(np.abs(view.read(gpu.OUT_GODRAY) - fr.godray)))
+    print(f"[parity] godray fix={fix}: max_abs={err}", flush=True)
+    assert err <= COLOR_ABS_TOL

@@ -1,0 +1,26 @@
+// vx_internal.h — host-side helpers shared by the library's translation units.
+#pragma once
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+
+#include "voxhip.h"
+
+static_assert(sizeof(vx_node) == 16, "NodeGpu is 16 bytes (gpu_types.rs:15-20)");
+static_assert(sizeof(vx_node_ropes) == 32, "NodeRopesGpu is 32 bytes (gpu_types.rs:24-33)");
+static_assert(sizeof(vx_chunk_meta) == 48, "ChunkMetaGpu is 48 bytes (gpu_types.rs:38-48)");
+static_assert(sizeof(vx_camera) == 464, "CameraGpu is 464 bytes (gpu_types.rs:52-86)");
+static_assert(sizeof(vx_clipmap) == 528, "ClipmapGpu is 528 bytes (gpu_types.rs:105-113)");
+static_assert(offsetof(vx_camera, cam_pos) == 256, "CameraGpu.cam_pos @256");
+static_assert(offsetof(vx_camera, chunk_size) == 320, "CameraGpu.chunk_size @320");
+static_assert(offsetof(vx_camera, voxel_params) == 336, "CameraGpu.voxel_params @336");
+static_assert(offsetof(vx_camera, grid_origin_chunk) == 352, "CameraGpu.grid_origin_chunk @352");
+static_assert(offsetof(vx_camera, render_present_px) == 384, "CameraGpu.render_present_px @384");
+static_assert(offsetof(vx_camera, profile_flags) == 400, "CameraGpu.profile_flags @400");
+static_assert(offsetof(vx_clipmap, offset) == 272, "ClipmapGpu.offset @272");
+static_assert(sizeof(vx_hit) == 32, "vx_hit is 32 bytes");
+static_assert(sizeof(vx_ray) == 32, "vx_ray is 32 bytes");
+
+// Records a message for vx_last_error() on this thread and returns `code`.
+int vx_fail(int code, const char* fmt, ...);

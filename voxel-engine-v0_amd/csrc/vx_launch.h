@@ -1,0 +1,14 @@
+// vx_launch.h — host-side interface between the runtime (vx_runtime.hip) and
+// the kernel launchers (vx_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "vxd_scene.h"
+
+hipError_t vx_launch_primary(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
+                             unsigned long long* counters, hipStream_t st);
+hipError_t vx_launch_local_taa(const vxd::FrameBuffers& fb, hipStream_t st);
+hipError_t vx_launch_godray(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
+                            unsigned long long* counters, hipStream_t st);
+hipError_t vx_launch_composite(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, hipStream_t st);
+hipError_t vx_launch_composite_taa(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, hipStream_t st);

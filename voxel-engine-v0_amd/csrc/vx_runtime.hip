@@ -1,0 +1,526 @@
+// vx_runtime.hip — the renderer boundary of libvoxhip (C ABI in voxhip.h).
+//
+// Replaces render::Renderer (render/state/mod.rs:34-712): owns the resident
+// world arenas and per-view history in HBM, applies region uploads, runs the
+// pass sequence of Renderer::encode_compute (render/state/mod.rs:329-446) as
+// stream-ordered kernel launches, and times each pass with HIP events.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "vx_internal.h"
+#include "vx_launch.h"
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+int vx_fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define VX_HIP(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return vx_fail(VX_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+struct vx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t flags = 0;
+  uint64_t node_cap = 0;
+  uint32_t chunk_cap = 0, grid_cap = 0, clip_levels = 0, clip_res = 0;
+  vx_node* nodes = nullptr;
+  vx_node_ropes* ropes = nullptr;
+  vx_chunk_meta* meta = nullptr;
+  uint32_t* macro = nullptr;
+  uint32_t* colinfo = nullptr;
+  uint32_t* grid = nullptr;
+  float* clip_height = nullptr;
+  vx_clipmap* clip_dev = nullptr;
+};
+
+struct vx_view {
+  vx_ctx* ctx = nullptr;
+  uint32_t pw = 0, ph = 0, rw = 0, rh = 0;
+  int ping = 0;
+  vx_camera* cam_dev = nullptr;
+  float4* color = nullptr;
+  float* depth = nullptr;
+  float4* local = nullptr;
+  float4* primary_hist[2] = {nullptr, nullptr};
+  float* shadow[2] = {nullptr, nullptr};
+  float4* local_hist[2] = {nullptr, nullptr};
+  float4* godray[2] = {nullptr, nullptr};
+  float4* pre_taa = nullptr;
+  float4* output = nullptr;
+  float4* out_hist[2] = {nullptr, nullptr};
+  vx_hit* hits = nullptr;
+  unsigned long long* counters = nullptr;
+  // Slot written last by each history-producing pass.
+  int written_primary = 1, written_local = 1, written_godray = 1, written_out = 1;
+  hipEvent_t ev[10] = {};
+  uint32_t last_mask = 0;
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = 0;
+  explicit DeviceGuard(int dev) {
+    hipGetDevice(&prev);
+    hipSetDevice(dev);
+  }
+  ~DeviceGuard() { hipSetDevice(prev); }
+};
+
+template <class T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+  if (e != hipSuccess) return vx_fail(VX_E_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+  e = hipMemset(*p, 0, count * sizeof(T));
+  if (e != hipSuccess) return vx_fail(VX_E_DEVICE, "hipMemset: %s", hipGetErrorString(e));
+  return VX_OK;
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+vxd::Scene make_scene(const vx_ctx* c, bool godray) {
+  vxd::Scene S;
+  S.chunks = c->meta;
+  S.nodes = reinterpret_cast<const uint4*>(c->nodes);
+  S.ropes = reinterpret_cast<const uint4*>(c->ropes);
+  S.grid = c->grid;
+  S.macro = c->macro;
+  S.colinfo = c->colinfo;
+  S.clip = c->clip_dev;
+  S.clip_height = c->clip_height;
+  S.n_chunks = c->chunk_cap;
+  S.n_nodes = c->node_cap;
+  S.n_grid = c->grid_cap;
+  S.n_macro = (uint64_t)c->chunk_cap * 16u;
+  S.n_colinfo = (uint64_t)c->chunk_cap * 2048u;
+  S.n_clip = (uint64_t)c->clip_levels * c->clip_res * c->clip_res;
+  if (godray && !(c->flags & VX_CTX_GODRAY_MACRO_FIX)) {
+    // The reference's godray pass sees the rope buffer where it expects macro_occ.
+    S.macro = reinterpret_cast<const uint32_t*>(c->ropes);
+    S.n_macro = c->node_cap * 8u;
+  }
+  return S;
+}
+
+void free_view_buffers(vx_view* v) {
+  dfree(v->color);
+  dfree(v->depth);
+  dfree(v->local);
+  for (int i = 0; i < 2; ++i) {
+    dfree(v->primary_hist[i]);
+    dfree(v->shadow[i]);
+    dfree(v->local_hist[i]);
+    dfree(v->godray[i]);
+    dfree(v->out_hist[i]);
+  }
+  dfree(v->pre_taa);
+  dfree(v->output);
+  dfree(v->hits);
+}
+
+int alloc_view_buffers(vx_view* v) {
+  const size_t r = (size_t)v->rw * v->rh, p = (size_t)v->pw * v->ph;
+  int rc = VX_OK;
+  if ((rc = dalloc(&v->color, r)) || (rc = dalloc(&v->depth, r)) || (rc = dalloc(&v->local, r))) return rc;
+  for (int i = 0; i < 2; ++i) {
+    if ((rc = dalloc(&v->primary_hist[i], r)) || (rc = dalloc(&v->shadow[i], r)) || (rc = dalloc(&v->local_hist[i], r)) ||
+        (rc = dalloc(&v->godray[i], r)) || (rc = dalloc(&v->out_hist[i], p)))
+      return rc;
+  }
+  if ((rc = dalloc(&v->pre_taa, p)) || (rc = dalloc(&v->output, p)) || (rc = dalloc(&v->hits, r))) return rc;
+  v->ping = 0;
+  v->written_primary = v->written_local = v->written_godray = v->written_out = 1;
+  return VX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vx_last_error(void) { return g_err; }
+
+const char* vx_version(void) {
+  return "libvoxhip 0.1 (hipcc " __VERSION__ ", gfx950, -O3 -ffp-contract=off, correctly rounded fp32 div/sqrt)";
+}
+
+int vx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int vx_create(const vx_caps* caps, vx_ctx** out) {
+  if (!out) return vx_fail(VX_E_INVALID, "vx_create: out is null");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return vx_fail(VX_E_NODEVICE, "vx_create: no GPU");
+  vx_caps c{};
+  if (caps) c = *caps;
+  if (c.device < 0 || c.device >= ndev) return vx_fail(VX_E_INVALID, "vx_create: bad device %d", c.device);
+  vx_ctx* ctx = new (std::nothrow) vx_ctx();
+  if (!ctx) return vx_fail(VX_E_NOMEM, "vx_create: host allocation");
+  ctx->device = c.device;
+  ctx->flags = c.flags;
+  ctx->node_cap = c.node_capacity ? c.node_capacity : (1024ull * 1024 * 1024) / sizeof(vx_node);  // buffers.rs:79
+  ctx->chunk_cap = c.chunk_capacity ? c.chunk_capacity : 57u * 4u * 57u;                           // buffers.rs:87-96
+  ctx->grid_cap = c.grid_capacity ? c.grid_capacity : 57u * 4u * 57u;
+  ctx->clip_levels = c.clip_levels ? c.clip_levels : VX_CLIP_LEVELS_MAX;
+  ctx->clip_res = c.clip_res ? c.clip_res : 256u;
+  if (ctx->node_cap > 0xFFFFFFFFull || ctx->clip_levels > VX_CLIP_LEVELS_MAX) {
+    delete ctx;
+    return vx_fail(VX_E_INVALID, "vx_create: capacity out of range");
+  }
+  DeviceGuard g(ctx->device);
+  int rc = VX_OK;
+  if ((rc = dalloc(&ctx->nodes, ctx->node_cap)) || (rc = dalloc(&ctx->ropes, ctx->node_cap)) ||
+      (rc = dalloc(&ctx->meta, ctx->chunk_cap)) || (rc = dalloc(&ctx->macro, (size_t)ctx->chunk_cap * 16)) ||
+      (rc = dalloc(&ctx->colinfo, (size_t)ctx->chunk_cap * 2048)) || (rc = dalloc(&ctx->grid, ctx->grid_cap)) ||
+      (rc = dalloc(&ctx->clip_height, (size_t)ctx->clip_levels * ctx->clip_res * ctx->clip_res)) ||
+      (rc = dalloc(&ctx->clip_dev, 1))) {
+    vx_destroy(ctx);
+    return rc;
+  }
+  vx_clipmap empty;
+  std::memset(&empty, 0, sizeof(empty));
+  empty.res = ctx->clip_res;
+  if (hipMemcpy(ctx->clip_dev, &empty, sizeof(empty), hipMemcpyHostToDevice) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    vx_destroy(ctx);
+    return vx_fail(VX_E_DEVICE, "vx_create: stream/copy failed");
+  }
+  *out = ctx;
+  return VX_OK;
+}
+
+void vx_destroy(vx_ctx* ctx) {
+  if (!ctx) return;
+  DeviceGuard g(ctx->device);
+  if (ctx->stream) {
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+  }
+  dfree(ctx->nodes);
+  dfree(ctx->ropes);
+  dfree(ctx->meta);
+  dfree(ctx->macro);
+  dfree(ctx->colinfo);
+  dfree(ctx->grid);
+  dfree(ctx->clip_height);
+  dfree(ctx->clip_dev);
+  delete ctx;
+}
+
+int vx_upload_regions(vx_ctx* ctx, int arena, const vx_region* regions, uint32_t n) {
+  if (!ctx || (!regions && n)) return vx_fail(VX_E_INVALID, "vx_upload_regions: null argument");
+  char* base = nullptr;
+  size_t esz = 0;
+  uint64_t cap = 0;
+  switch (arena) {
+    case VX_ARENA_NODES: base = (char*)ctx->nodes; esz = sizeof(vx_node); cap = ctx->node_cap; break;
+    case VX_ARENA_ROPES: base = (char*)ctx->ropes; esz = sizeof(vx_node_ropes); cap = ctx->node_cap; break;
+    case VX_ARENA_META: base = (char*)ctx->meta; esz = sizeof(vx_chunk_meta); cap = ctx->chunk_cap; break;
+    case VX_ARENA_MACRO: base = (char*)ctx->macro; esz = 4; cap = (uint64_t)ctx->chunk_cap * 16; break;
+    case VX_ARENA_COLINFO: base = (char*)ctx->colinfo; esz = 4; cap = (uint64_t)ctx->chunk_cap * 2048; break;
+    default: return vx_fail(VX_E_INVALID, "vx_upload_regions: bad arena %d", arena);
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_region& r = regions[i];
+    if (r.count == 0) continue;
+    if (!r.data) return vx_fail(VX_E_INVALID, "vx_upload_regions: region %u has no data", i);
+    if (r.offset > cap || r.count > cap - r.offset)
+      return vx_fail(VX_E_CAPACITY, "vx_upload_regions: region %u [%llu, +%llu) exceeds capacity %llu", i,
+                     (unsigned long long)r.offset, (unsigned long long)r.count, (unsigned long long)cap);
+  }
+  DeviceGuard g(ctx->device);
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_region& r = regions[i];
+    if (r.count == 0) continue;
+    VX_HIP(hipMemcpyAsync(base + r.offset * esz, r.data, r.count * esz, hipMemcpyHostToDevice, ctx->stream));
+  }
+  VX_HIP(hipStreamSynchronize(ctx->stream));
+  return VX_OK;
+}
+
+int vx_write_grid(vx_ctx* ctx, const uint32_t* grid, uint32_t cells) {
+  if (!ctx || (!grid && cells)) return vx_fail(VX_E_INVALID, "vx_write_grid: null argument");
+  if (cells > ctx->grid_cap) return vx_fail(VX_E_CAPACITY, "vx_write_grid: %u cells > capacity %u", cells, ctx->grid_cap);
+  DeviceGuard g(ctx->device);
+  VX_HIP(hipMemcpyAsync(ctx->grid, grid, (size_t)cells * 4, hipMemcpyHostToDevice, ctx->stream));
+  VX_HIP(hipStreamSynchronize(ctx->stream));
+  return VX_OK;
+}
+
+int vx_write_clipmap(vx_ctx* ctx, const vx_clipmap* params, const vx_clip_patch* patches, uint32_t n) {
+  if (!ctx || !params || (!patches && n)) return vx_fail(VX_E_INVALID, "vx_write_clipmap: null argument");
+  if (params->levels > ctx->clip_levels || (params->levels && params->res != ctx->clip_res))
+    return vx_fail(VX_E_CAPACITY, "vx_write_clipmap: levels/res (%u, %u) do not fit (%u, %u)", params->levels, params->res,
+                   ctx->clip_levels, ctx->clip_res);
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_clip_patch& p = patches[i];
+    if (p.level >= ctx->clip_levels || p.x + p.w > ctx->clip_res || p.y + p.h > ctx->clip_res || (!p.data && p.w && p.h))
+      return vx_fail(VX_E_INVALID, "vx_write_clipmap: patch %u out of range", i);
+  }
+  DeviceGuard g(ctx->device);
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_clip_patch& p = patches[i];
+    if (!p.w || !p.h) continue;
+    float* dst = ctx->clip_height + ((size_t)p.level * ctx->clip_res + p.y) * ctx->clip_res + p.x;
+    VX_HIP(hipMemcpy2DAsync(dst, (size_t)ctx->clip_res * 4, p.data, (size_t)p.w * 4, (size_t)p.w * 4, p.h, hipMemcpyHostToDevice,
+                            ctx->stream));
+  }
+  VX_HIP(hipMemcpyAsync(ctx->clip_dev, params, sizeof(vx_clipmap), hipMemcpyHostToDevice, ctx->stream));
+  VX_HIP(hipStreamSynchronize(ctx->stream));
+  return VX_OK;
+}
+
+int vx_view_create(vx_ctx* ctx, uint32_t pw, uint32_t ph, uint32_t rw, uint32_t rh, vx_view** out) {
+  if (!ctx || !out || !pw || !ph || !rw || !rh) return vx_fail(VX_E_INVALID, "vx_view_create: bad argument");
+  if ((uint64_t)pw * ph > (1ull << 28) || (uint64_t)rw * rh > (1ull << 28)) return vx_fail(VX_E_INVALID, "vx_view_create: too large");
+  *out = nullptr;
+  vx_view* v = new (std::nothrow) vx_view();
+  if (!v) return vx_fail(VX_E_NOMEM, "vx_view_create: host allocation");
+  v->ctx = ctx;
+  v->pw = pw;
+  v->ph = ph;
+  v->rw = rw;
+  v->rh = rh;
+  DeviceGuard g(ctx->device);
+  int rc = alloc_view_buffers(v);
+  if (!rc) rc = dalloc(&v->cam_dev, 1);
+  if (!rc) rc = dalloc(&v->counters, 16);
+  for (int i = 0; i < 10 && !rc; ++i)
+    if (hipEventCreate(&v->ev[i]) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "vx_view_create: hipEventCreate");
+  if (rc) {
+    vx_view_destroy(v);
+    return rc;
+  }
+  *out = v;
+  return VX_OK;
+}
+
+void vx_view_destroy(vx_view* v) {
+  if (!v) return;
+  DeviceGuard g(v->ctx->device);
+  hipStreamSynchronize(v->ctx->stream);
+  free_view_buffers(v);
+  dfree(v->cam_dev);
+  dfree(v->counters);
+  for (int i = 0; i < 10; ++i)
+    if (v->ev[i]) hipEventDestroy(v->ev[i]);
+  delete v;
+}
+
+int vx_view_resize(vx_view* v, uint32_t pw, uint32_t ph, uint32_t rw, uint32_t rh) {
+  if (!v || !pw || !ph || !rw || !rh) return vx_fail(VX_E_INVALID, "vx_view_resize: bad argument");
+  DeviceGuard g(v->ctx->device);
+  VX_HIP(hipStreamSynchronize(v->ctx->stream));
+  free_view_buffers(v);
+  v->pw = pw;
+  v->ph = ph;
+  v->rw = rw;
+  v->rh = rh;
+  return alloc_view_buffers(v);
+}
+
+int vx_view_reset(vx_view* v) {
+  if (!v) return vx_fail(VX_E_INVALID, "vx_view_reset: null view");
+  DeviceGuard g(v->ctx->device);
+  const size_t r = (size_t)v->rw * v->rh, p = (size_t)v->pw * v->ph;
+  hipStream_t s = v->ctx->stream;
+  for (int i = 0; i < 2; ++i) {
+    VX_HIP(hipMemsetAsync(v->primary_hist[i], 0, r * 16, s));
+    VX_HIP(hipMemsetAsync(v->shadow[i], 0, r * 4, s));
+    VX_HIP(hipMemsetAsync(v->local_hist[i], 0, r * 16, s));
+    VX_HIP(hipMemsetAsync(v->godray[i], 0, r * 16, s));
+    VX_HIP(hipMemsetAsync(v->out_hist[i], 0, p * 16, s));
+  }
+  VX_HIP(hipStreamSynchronize(s));
+  v->ping = 0;
+  v->written_primary = v->written_local = v->written_godray = v->written_out = 1;
+  return VX_OK;
+}
+
+int vx_write_camera(vx_view* v, const vx_camera* cam) {
+  if (!v || !cam) return vx_fail(VX_E_INVALID, "vx_write_camera: null argument");
+  DeviceGuard g(v->ctx->device);
+  VX_HIP(hipMemcpyAsync(v->cam_dev, cam, sizeof(vx_camera), hipMemcpyHostToDevice, v->ctx->stream));
+  VX_HIP(hipStreamSynchronize(v->ctx->stream));
+  return VX_OK;
+}
+
+int vx_render(vx_view* v, uint32_t mask, void* stream) {
+  if (!v) return vx_fail(VX_E_INVALID, "vx_render: null view");
+  vx_ctx* c = v->ctx;
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  DeviceGuard g(c->device);
+  const int ping = v->ping, pong = 1 - ping;
+  vxd::FrameBuffers fb;
+  std::memset(&fb, 0, sizeof(fb));
+  fb.rw = v->rw;
+  fb.rh = v->rh;
+  fb.pw = v->pw;
+  fb.ph = v->ph;
+  fb.primary_hist_in = v->primary_hist[ping];
+  fb.primary_hist_out = v->primary_hist[pong];
+  fb.shadow_in = v->shadow[ping];
+  fb.shadow_out = v->shadow[pong];
+  fb.color = v->color;
+  fb.depth = v->depth;
+  fb.local = v->local;
+  fb.local_hist_in = v->local_hist[ping];
+  fb.local_hist_out = v->local_hist[pong];
+  fb.godray_hist = v->godray[ping];
+  fb.godray_out = v->godray[pong];
+  fb.pre_taa = v->pre_taa;
+  fb.out_hist_in = v->out_hist[ping];
+  fb.out_hist_out = v->out_hist[pong];
+  fb.output = v->output;
+  fb.hits = (mask & VX_PASS_WRITE_HITS) ? v->hits : nullptr;
+  const bool count = (mask & VX_PASS_COUNT) != 0;
+  const vxd::Scene S = make_scene(c, false);
+  VX_HIP(hipEventRecord(v->ev[0], st));
+  if (mask & VX_PASS_PRIMARY) {
+    VX_HIP(vx_launch_primary(S, v->cam_dev, fb, count, v->counters, st));
+    v->written_primary = pong;
+  }
+  VX_HIP(hipEventRecord(v->ev[1], st));
+  if (mask & VX_PASS_LOCAL_TAA) {
+    VX_HIP(vx_launch_local_taa(fb, st));
+    v->written_local = pong;
+  }
+  VX_HIP(hipEventRecord(v->ev[2], st));
+  if (mask & VX_PASS_GODRAY) {
+    VX_HIP(vx_launch_godray(make_scene(c, true), v->cam_dev, fb, count, v->counters, st));
+    v->written_godray = pong;
+  }
+  VX_HIP(hipEventRecord(v->ev[3], st));
+  if (mask & VX_PASS_COMPOSITE) VX_HIP(vx_launch_composite(S, v->cam_dev, fb, st));
+  VX_HIP(hipEventRecord(v->ev[4], st));
+  if (mask & VX_PASS_COMPOSITE_TAA) {
+    VX_HIP(vx_launch_composite_taa(S, v->cam_dev, fb, st));
+    v->written_out = pong;
+  }
+  VX_HIP(hipEventRecord(v->ev[5], st));
+  v->last_mask = mask;
+  if (mask & VX_PASS_END_FRAME) v->ping = pong;
+  return VX_OK;
+}
+
+static int output_buffer(vx_view* v, int which, void** p, size_t* bytes) {
+  const size_t r = (size_t)v->rw * v->rh, pp = (size_t)v->pw * v->ph;
+  switch (which) {
+    case VX_OUT_COLOR: *p = v->color; *bytes = r * 16; return VX_OK;
+    case VX_OUT_DEPTH: *p = v->depth; *bytes = r * 4; return VX_OK;
+    case VX_OUT_LOCAL: *p = v->local; *bytes = r * 16; return VX_OK;
+    case VX_OUT_PRIMARY_HIST: *p = v->primary_hist[v->written_primary]; *bytes = r * 16; return VX_OK;
+    case VX_OUT_SHADOW_HIST: *p = v->shadow[v->written_primary]; *bytes = r * 4; return VX_OK;
+    case VX_OUT_LOCAL_HIST: *p = v->local_hist[v->written_local]; *bytes = r * 16; return VX_OK;
+    case VX_OUT_GODRAY: *p = v->godray[v->written_godray]; *bytes = r * 16; return VX_OK;
+    case VX_OUT_PRE_TAA: *p = v->pre_taa; *bytes = pp * 16; return VX_OK;
+    case VX_OUT_OUTPUT: *p = v->output; *bytes = pp * 16; return VX_OK;
+    case VX_OUT_HITS: *p = v->hits; *bytes = r * sizeof(vx_hit); return VX_OK;
+    default: return vx_fail(VX_E_INVALID, "bad output selector %d", which);
+  }
+}
+
+int vx_read_output(vx_view* v, int which, void* dst, size_t bytes) {
+  if (!v || !dst) return vx_fail(VX_E_INVALID, "vx_read_output: null argument");
+  void* p = nullptr;
+  size_t n = 0;
+  int rc = output_buffer(v, which, &p, &n);
+  if (rc) return rc;
+  if (bytes < n) return vx_fail(VX_E_INVALID, "vx_read_output: need %zu bytes, got %zu", n, bytes);
+  DeviceGuard g(v->ctx->device);
+  VX_HIP(hipStreamSynchronize(v->ctx->stream));
+  VX_HIP(hipDeviceSynchronize());
+  VX_HIP(hipMemcpy(dst, p, n, hipMemcpyDeviceToHost));
+  return VX_OK;
+}
+
+int vx_output_device_ptr(vx_view* v, int which, void** dptr, size_t* bytes) {
+  if (!v || !dptr || !bytes) return vx_fail(VX_E_INVALID, "vx_output_device_ptr: null argument");
+  return output_buffer(v, which, dptr, bytes);
+}
+
+int vx_write_history(vx_view* v, int which, const void* src, size_t bytes) {
+  if (!v || !src) return vx_fail(VX_E_INVALID, "vx_write_history: null argument");
+  const size_t r = (size_t)v->rw * v->rh, pp = (size_t)v->pw * v->ph;
+  void* dst = nullptr;
+  size_t n = 0;
+  const int ping = v->ping;  // the slot the next frame reads
+  switch (which) {
+    case VX_OUT_PRIMARY_HIST: dst = v->primary_hist[ping]; n = r * 16; break;
+    case VX_OUT_SHADOW_HIST: dst = v->shadow[ping]; n = r * 4; break;
+    case VX_OUT_LOCAL_HIST: dst = v->local_hist[ping]; n = r * 16; break;
+    case VX_OUT_GODRAY: dst = v->godray[ping]; n = r * 16; break;
+    case VX_OUT_OUTPUT: dst = v->out_hist[ping]; n = pp * 16; break;
+    default: return vx_fail(VX_E_INVALID, "vx_write_history: selector %d is not a history", which);
+  }
+  if (bytes != n) return vx_fail(VX_E_INVALID, "vx_write_history: need %zu bytes, got %zu", n, bytes);
+  DeviceGuard g(v->ctx->device);
+  VX_HIP(hipMemcpy(dst, src, n, hipMemcpyHostToDevice));
+  return VX_OK;
+}
+
+int vx_view_counters(vx_view* v, uint64_t out[9], int reset) {
+  if (!v || !out) return vx_fail(VX_E_INVALID, "vx_view_counters: null argument");
+  DeviceGuard g(v->ctx->device);
+  VX_HIP(hipDeviceSynchronize());
+  unsigned long long tmp[16];
+  VX_HIP(hipMemcpy(tmp, v->counters, sizeof(tmp), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 9; ++i) out[i] = tmp[i];
+  if (reset) VX_HIP(hipMemset(v->counters, 0, sizeof(tmp)));
+  return VX_OK;
+}
+
+int vx_view_timings(vx_view* v, float ms[5]) {
+  if (!v || !ms) return vx_fail(VX_E_INVALID, "vx_view_timings: null argument");
+  DeviceGuard g(v->ctx->device);
+  VX_HIP(hipEventSynchronize(v->ev[5]));
+  for (int i = 0; i < 5; ++i) {
+    float t = 0.0f;
+    VX_HIP(hipEventElapsedTime(&t, v->ev[i], v->ev[i + 1]));
+    ms[i] = t;
+  }
+  return VX_OK;
+}
+
+int vx_sync(vx_ctx* ctx) {
+  if (!ctx) return vx_fail(VX_E_INVALID, "vx_sync: null context");
+  DeviceGuard g(ctx->device);
+  VX_HIP(hipStreamSynchronize(ctx->stream));
+  return VX_OK;
+}
+
+int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n, vx_hit* hits_dev, void* stream) {
+  (void)ctx;
+  (void)scene_cam;
+  (void)rays_dev;
+  (void)n;
+  (void)hits_dev;
+  (void)stream;
+  return vx_fail(VX_E_INVALID, "vx_query_rays: not available in this build");
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// vxd_scene.h — device-side view of the resident world and the bound camera,
+// plus the per-load algorithmic byte counters used by the counting variant
+// of the kernels.
+#pragma once
+#include "voxhip.h"
+#include "vxd_math.h"
+
+namespace vxd {
+
+// Resident world, as uploaded through the C ABI (render/gpu_types.rs formats).
+struct Scene {
+  const vx_chunk_meta* __restrict__ chunks;
+  const uint4* __restrict__ nodes;    // vx_node as one 16-byte vector
+  const uint4* __restrict__ ropes;    // vx_node_ropes as two 16-byte vectors
+  const uint32_t* __restrict__ grid;
+  const uint32_t* __restrict__ macro;  // macro occupancy words (or rope words, godray compat)
+  const uint32_t* __restrict__ colinfo;
+  const vx_clipmap* __restrict__ clip;
+  const float* __restrict__ clip_height;
+  uint64_t n_chunks, n_nodes, n_grid, n_macro, n_colinfo, n_clip;
+};
+
+// Algorithmic loads by array (element counts; bytes = count * element size).
+struct Counts {
+  uint32_t grid, meta, macro, node, rope, colinfo, clip, tex_rd, tex_wr;
+};
+constexpr int kNumCounts = 9;
+
+}  // namespace vxd
+
+namespace vxd {
+
+// Per-view textures of one frame, as linear row-major arrays (render res for
+// primary/local TAA/godray, present res for composite/composite TAA).
+struct FrameBuffers {
+  uint32_t rw, rh, pw, ph;
+  const float4* __restrict__ primary_hist_in;
+  float4* __restrict__ primary_hist_out;
+  const float* __restrict__ shadow_in;
+  float* __restrict__ shadow_out;
+  float4* __restrict__ color;
+  float* __restrict__ depth;
+  float4* __restrict__ local;
+  const float4* __restrict__ local_hist_in;
+  float4* __restrict__ local_hist_out;  // written by local TAA, read by composite
+  const float4* __restrict__ godray_hist;
+  float4* __restrict__ godray_out;      // written by godray, read by composite
+  float4* __restrict__ pre_taa;
+  const float4* __restrict__ out_hist_in;
+  float4* __restrict__ out_hist_out;
+  float4* __restrict__ output;
+  vx_hit* __restrict__ hits;
+};
+
+// Workgroup-shared primary candidate list (common.wgsl:422-424).
+constexpr uint32_t kMaxTileChunks = 192;
+struct TileLDS {
+  uint32_t count;
+  uint32_t cached;
+  uint32_t slots[kMaxTileChunks];
+  float enter[kMaxTileChunks];
+};
+
+}  // namespace vxd

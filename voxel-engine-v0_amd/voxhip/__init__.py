@@ -1,0 +1,439 @@
+"""voxhip — Python bindings of libvoxhip, the MI355X implementation of the
+voxel-engine-v0 primary / local TAA / godray / composite passes.
+
+The bindings mirror the reference renderer boundary (render::Renderer,
+src/render/state/mod.rs:34-712): a Context owns the resident world
+(Renderer::new / apply_chunk_uploads / write_chunk_grid /
+write_clipmap_updates), a View owns one camera stream and its history
+(write_camera / encode_compute / resize_output).  All work runs in
+libvoxhip.so; this module only moves arguments across the C ABI.  Loading
+fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libvoxhip.so"
+
+VX_OK = 0
+INVALID = 0xFFFFFFFF
+MAT_AIR, MAT_GRASS, MAT_DIRT, MAT_STONE, MAT_WOOD, MAT_LEAF, MAT_LIGHT = range(7)
+MAX_CHUNK_NODES = 299593
+
+PASS_PRIMARY = 0x01
+PASS_LOCAL_TAA = 0x02
+PASS_GODRAY = 0x04
+PASS_COMPOSITE = 0x08
+PASS_COMPOSITE_TAA = 0x10
+PASS_END_FRAME = 0x80
+PASS_FRAME = 0x9F
+PASS_WRITE_HITS = 0x100
+PASS_COUNT = 0x200
+
+OUT_COLOR, OUT_DEPTH, OUT_LOCAL, OUT_PRIMARY_HIST, OUT_SHADOW_HIST = 0, 1, 2, 3, 4
+OUT_LOCAL_HIST, OUT_GODRAY, OUT_PRE_TAA, OUT_OUTPUT, OUT_HITS = 5, 6, 7, 8, 9
+
+ARENA_NODES, ARENA_ROPES, ARENA_META, ARENA_MACRO, ARENA_COLINFO = range(5)
+CTX_GODRAY_MACRO_FIX = 1
+CAM_JITTER = 1
+
+NODE_DTYPE = np.dtype([("child_base", "<u4"), ("child_mask", "<u4"), ("material", "<u4"), ("key", "<u4")])
+ROPES_DTYPE = np.dtype([(n, "<u4") for n in ("px", "nx", "py", "ny", "pz", "nz", "pad0", "pad1")])
+META_DTYPE = np.dtype(
+    [("origin", "<i4", (4,)), ("node_base", "<u4"), ("node_count", "<u4"), ("macro_base", "<u4"),
+     ("colinfo_base", "<u4"), ("macro_empty", "<u4"), ("pad", "<u4", (3,))]
+)
+HIT_DTYPE = np.dtype(
+    [("slot", "<u4"), ("node", "<u4"), ("voxel", "<i4", (3,)), ("mat", "<u4"), ("t", "<f4"), ("kind", "<u4")]
+)
+COUNT_NAMES = ("grid", "meta", "macro", "node", "rope", "colinfo", "clip", "tex_rd", "tex_wr")
+# Bytes per counted element (gpu_types.rs struct sizes; textures count bytes).
+COUNT_BYTES = (4, 48, 4, 16, 32, 4, 4, 1, 1)
+
+assert NODE_DTYPE.itemsize == 16 and ROPES_DTYPE.itemsize == 32 and META_DTYPE.itemsize == 48
+assert HIT_DTYPE.itemsize == 32
+
+
+class Camera(C.Structure):
+    """CameraGpu, src/render/gpu_types.rs:52-86 (464 bytes)."""
+
+    _fields_ = [
+        ("view_inv", C.c_float * 16), ("proj_inv", C.c_float * 16), ("view_proj", C.c_float * 16),
+        ("prev_view_proj", C.c_float * 16), ("cam_pos", C.c_float * 4), ("ray00", C.c_float * 4),
+        ("ray_dx", C.c_float * 4), ("ray_dy", C.c_float * 4), ("chunk_size", C.c_uint32),
+        ("chunk_count", C.c_uint32), ("max_steps", C.c_uint32), ("frame_index", C.c_uint32),
+        ("voxel_params", C.c_float * 4), ("grid_origin_chunk", C.c_int32 * 4), ("grid_dims", C.c_uint32 * 4),
+        ("render_present_px", C.c_uint32 * 4), ("profile_flags", C.c_uint32), ("pad_profile", C.c_uint32 * 3),
+        ("pad0", C.c_uint32 * 4), ("pad1", C.c_uint32 * 4), ("pad2", C.c_uint32 * 4),
+    ]
+
+
+class Clipmap(C.Structure):
+    """ClipmapGpu, src/render/gpu_types.rs:105-113 (528 bytes)."""
+
+    _fields_ = [("levels", C.c_uint32), ("res", C.c_uint32), ("base_cell_m", C.c_float), ("pad0", C.c_float),
+                ("level", (C.c_float * 4) * 16), ("offset", (C.c_uint32 * 4) * 16)]
+
+
+class Caps(C.Structure):
+    _fields_ = [("node_capacity", C.c_uint64), ("chunk_capacity", C.c_uint32), ("grid_capacity", C.c_uint32),
+                ("clip_levels", C.c_uint32), ("clip_res", C.c_uint32), ("device", C.c_int32), ("flags", C.c_uint32)]
+
+
+class Region(C.Structure):
+    _fields_ = [("offset", C.c_uint64), ("count", C.c_uint64), ("data", C.c_void_p)]
+
+
+class ClipPatch(C.Structure):
+    _fields_ = [("level", C.c_uint32), ("x", C.c_uint32), ("y", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
+                ("data", C.c_void_p)]
+
+
+class WorldParams(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("base_y", C.c_int32), ("amplitude", C.c_int32), ("dirt_depth", C.c_int32),
+                ("tree_density", C.c_float), ("trunk_min", C.c_int32), ("trunk_max", C.c_int32),
+                ("leaf_radius", C.c_int32), ("wavelength0", C.c_int32)]
+
+
+class Pose(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("yaw", C.c_float), ("pitch", C.c_float), ("fovy", C.c_float),
+                ("z_near", C.c_float), ("z_far", C.c_float)]
+
+
+class HostWorldView(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("ropes", C.c_void_p), ("meta", C.c_void_p),
+                ("chunk_count", C.c_uint32), ("macro_words", C.c_void_p), ("colinfo_words", C.c_void_p),
+                ("grid", C.c_void_p), ("grid_cells", C.c_uint32), ("grid_origin", C.c_int32 * 3),
+                ("grid_dims", C.c_uint32 * 3)]
+
+
+assert C.sizeof(Camera) == 464 and C.sizeof(Clipmap) == 528
+
+
+class VoxhipError(RuntimeError):
+    pass
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise VoxhipError(f"{LIB_PATH} is missing: build it with `python voxel-engine-v0_amd/build.py`")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, u32, i32, u64, f32 = C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_float
+    sig = {
+        "vx_last_error": ([], C.c_char_p), "vx_version": ([], C.c_char_p), "vx_device_count": ([], C.c_int),
+        "vx_create": ([C.POINTER(Caps), C.POINTER(vp)], C.c_int), "vx_destroy": ([vp], None),
+        "vx_upload_regions": ([vp, C.c_int, C.POINTER(Region), u32], C.c_int),
+        "vx_write_grid": ([vp, vp, u32], C.c_int),
+        "vx_write_clipmap": ([vp, C.POINTER(Clipmap), C.POINTER(ClipPatch), u32], C.c_int),
+        "vx_view_create": ([vp, u32, u32, u32, u32, C.POINTER(vp)], C.c_int), "vx_view_destroy": ([vp], None),
+        "vx_view_resize": ([vp, u32, u32, u32, u32], C.c_int), "vx_view_reset": ([vp], C.c_int),
+        "vx_write_camera": ([vp, C.POINTER(Camera)], C.c_int), "vx_render": ([vp, u32, vp], C.c_int),
+        "vx_read_output": ([vp, C.c_int, vp, C.c_size_t], C.c_int),
+        "vx_output_device_ptr": ([vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_size_t)], C.c_int),
+        "vx_write_history": ([vp, C.c_int, vp, C.c_size_t], C.c_int),
+        "vx_view_counters": ([vp, C.POINTER(u64), C.c_int], C.c_int),
+        "vx_view_timings": ([vp, C.POINTER(f32)], C.c_int),
+        "vx_query_rays": ([vp, C.POINTER(Camera), vp, u32, vp, vp], C.c_int),
+        "vx_sync": ([vp], C.c_int),
+        "vx_build_chunk": ([vp, vp, i32, i32, vp, vp, u32, C.POINTER(u32), vp, vp], C.c_int),
+        "vx_world_default": ([C.POINTER(WorldParams)], None),
+        "vx_world_ground_height": ([C.POINTER(WorldParams), i32, i32], i32),
+        "vx_world_fill_chunk": ([C.POINTER(WorldParams), i32, i32, i32, vp, vp], C.c_int),
+        "vx_host_world_build": ([C.POINTER(WorldParams), C.POINTER(i32), C.POINTER(u32), u32, C.POINTER(vp)], C.c_int),
+        "vx_host_world_free": ([vp], None),
+        "vx_host_world_get": ([vp, C.POINTER(HostWorldView)], C.c_int),
+        "vx_host_world_edit": ([vp, vp, vp, u32, vp, C.POINTER(u32)], C.c_int),
+        "vx_camera_build": ([C.POINTER(Pose), u32, u32, u32, u32, u32, f32, f32, f32, C.POINTER(i32), C.POINTER(u32), u32,
+                             vp, u32, C.POINTER(Camera)], C.c_int),
+        "vx_clipmap_build": ([C.POINTER(WorldParams), f32, f32, u32, u32, f32, C.POINTER(Clipmap), vp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+LIB = _load()
+EXPORTED = (
+    "vx_last_error", "vx_version", "vx_device_count", "vx_create", "vx_destroy", "vx_upload_regions", "vx_write_grid",
+    "vx_write_clipmap", "vx_view_create", "vx_view_destroy", "vx_view_resize", "vx_view_reset", "vx_write_camera",
+    "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings",
+    "vx_query_rays", "vx_sync", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
+    "vx_host_world_build", "vx_host_world_free", "vx_host_world_get", "vx_host_world_edit", "vx_camera_build",
+    "vx_clipmap_build",
+)
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != VX_OK:
+        raise VoxhipError(f"{what} failed ({rc}): {LIB.vx_last_error().decode()}")
+
+
+def _ptr(a: np.ndarray) -> int:
+    assert a.flags.c_contiguous
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    return int(LIB.vx_device_count())
+
+
+# ---------------------------------------------------------------------------
+# Host producers
+# ---------------------------------------------------------------------------
+def world_params(seed: int = 42, **kw) -> WorldParams:
+    p = WorldParams()
+    LIB.vx_world_default(C.byref(p))
+    p.seed = seed
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def ground_height(p: WorldParams, wx: int, wz: int) -> int:
+    return int(LIB.vx_world_ground_height(C.byref(p), wx, wz))
+
+
+def fill_chunk(p: WorldParams, cx: int, cy: int, cz: int) -> tuple[np.ndarray, np.ndarray]:
+    mat = np.zeros(64 * 64 * 64, np.uint8)
+    ground = np.zeros(64 * 64, np.int32)
+    _check(LIB.vx_world_fill_chunk(C.byref(p), cx, cy, cz, _ptr(mat), _ptr(ground)), "vx_world_fill_chunk")
+    return mat, ground
+
+
+@dataclass
+class ChunkBuild:
+    nodes: np.ndarray
+    ropes: np.ndarray
+    macro: np.ndarray
+    colinfo: np.ndarray
+
+
+def build_chunk(material: np.ndarray, ground: np.ndarray, chunk_oy: int, dirt_depth: int) -> ChunkBuild:
+    material = np.ascontiguousarray(material, np.uint8).reshape(-1)
+    ground = np.ascontiguousarray(ground, np.int32).reshape(-1)
+    assert material.size == 64 ** 3 and ground.size == 64 * 64
+    nodes = np.zeros(MAX_CHUNK_NODES, NODE_DTYPE)
+    ropes = np.zeros(MAX_CHUNK_NODES, ROPES_DTYPE)
+    macro = np.zeros(16, np.uint32)
+    colinfo = np.zeros(2048, np.uint32)
+    n = C.c_uint32(0)
+    _check(LIB.vx_build_chunk(_ptr(material), _ptr(ground), chunk_oy, dirt_depth, _ptr(nodes), _ptr(ropes),
+                              MAX_CHUNK_NODES, C.byref(n), _ptr(macro), _ptr(colinfo)), "vx_build_chunk")
+    return ChunkBuild(nodes[: n.value].copy(), ropes[: n.value].copy(), macro, colinfo)
+
+
+class HostWorld:
+    """A resident world built on the host (every grid chunk through vx_build_chunk)."""
+
+    def __init__(self, params: WorldParams, grid_origin=(0, 0, 0), grid_dims=(8, 4, 8), threads: int = 0):
+        self.params = params
+        h = C.c_void_p()
+        go = (C.c_int32 * 3)(*grid_origin)
+        gd = (C.c_uint32 * 3)(*grid_dims)
+        _check(LIB.vx_host_world_build(C.byref(params), go, gd, threads, C.byref(h)), "vx_host_world_build")
+        self._h = h
+        self._refresh()
+
+    def _refresh(self) -> None:
+        v = HostWorldView()
+        _check(LIB.vx_host_world_get(self._h, C.byref(v)), "vx_host_world_get")
+
+        def arr(ptr, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            buf = (C.c_byte * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+            return np.frombuffer(buf, dtype=dtype, count=count)
+
+        self.nodes = arr(v.nodes, v.node_count, NODE_DTYPE)
+        self.ropes = arr(v.ropes, v.node_count, ROPES_DTYPE)
+        self.meta = arr(v.meta, v.chunk_count, META_DTYPE)
+        self.macro = arr(v.macro_words, v.chunk_count * 16, np.uint32)
+        self.colinfo = arr(v.colinfo_words, v.chunk_count * 2048, np.uint32)
+        self.grid = arr(v.grid, v.grid_cells, np.uint32)
+        self.grid_origin = tuple(v.grid_origin)
+        self.grid_dims = tuple(v.grid_dims)
+        self.chunk_count = int(v.chunk_count)
+
+    def edit(self, xyz: np.ndarray, mats: np.ndarray) -> list[int]:
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        mats = np.ascontiguousarray(mats, np.uint8).reshape(-1)
+        out = np.zeros(max(1, len(mats)), np.uint32)
+        n = C.c_uint32(0)
+        _check(LIB.vx_host_world_edit(self._h, _ptr(xyz), _ptr(mats), len(mats), _ptr(out), C.byref(n)),
+               "vx_host_world_edit")
+        self._refresh()
+        return [int(s) for s in out[: n.value]]
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_host_world_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def camera_build(pose: Pose, render_wh, present_wh, frame_index: int, *, time_s: float = 0.0,
+                 wind_strength: float = 2.0, fog_density: float = 0.002, grid_origin=(0, 0, 0),
+                 grid_dims=(8, 4, 8), chunk_count: int = 0, prev_view_proj=None, jitter: bool = False) -> Camera:
+    cam = Camera()
+    go = (C.c_int32 * 3)(*grid_origin)
+    gd = (C.c_uint32 * 3)(*grid_dims)
+    prev = None
+    if prev_view_proj is not None:
+        pv = np.ascontiguousarray(prev_view_proj, np.float32).reshape(16)
+        prev = pv.ctypes.data
+    _check(LIB.vx_camera_build(C.byref(pose), render_wh[0], render_wh[1], present_wh[0], present_wh[1], frame_index,
+                               time_s, wind_strength, fog_density, go, gd, chunk_count, prev,
+                               CAM_JITTER if jitter else 0, C.byref(cam)), "vx_camera_build")
+    return cam
+
+
+def make_pose(pos, yaw: float, pitch: float, fovy_deg: float = 70.0, z_near: float = 0.1, z_far: float = 1000.0) -> Pose:
+    p = Pose()
+    p.pos[0], p.pos[1], p.pos[2] = (float(v) for v in pos)
+    p.yaw, p.pitch, p.fovy = yaw, pitch, math.radians(fovy_deg)
+    p.z_near, p.z_far = z_near, z_far
+    return p
+
+
+def clipmap_build(params: WorldParams, cam_x: float, cam_z: float, levels: int = 5, res: int = 256,
+                  base_cell_m: float = 1.0) -> tuple[Clipmap, np.ndarray]:
+    clip = Clipmap()
+    heights = np.zeros((max(levels, 1), res, res), np.float32)
+    _check(LIB.vx_clipmap_build(C.byref(params), cam_x, cam_z, levels, res, base_cell_m, C.byref(clip), _ptr(heights)),
+           "vx_clipmap_build")
+    return clip, heights
+
+
+# ---------------------------------------------------------------------------
+# Renderer boundary
+# ---------------------------------------------------------------------------
+class Context:
+    """Resident world on one GPU (Renderer::new + uploads)."""
+
+    def __init__(self, node_capacity: int, chunk_capacity: int, grid_capacity: int, clip_levels: int = 5,
+                 clip_res: int = 256, device: int = 0, godray_macro_fix: bool = False):
+        caps = Caps(node_capacity, chunk_capacity, grid_capacity, clip_levels, clip_res, device,
+                    CTX_GODRAY_MACRO_FIX if godray_macro_fix else 0)
+        h = C.c_void_p()
+        _check(LIB.vx_create(C.byref(caps), C.byref(h)), "vx_create")
+        self._h = h
+        self.clip_levels, self.clip_res = clip_levels, clip_res
+
+    def upload(self, arena: int, array: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(array)
+        r = Region(offset, len(a), a.ctypes.data)
+        _check(LIB.vx_upload_regions(self._h, arena, C.byref(r), 1), "vx_upload_regions")
+
+    def upload_world(self, w: HostWorld) -> None:
+        self.upload(ARENA_NODES, w.nodes)
+        self.upload(ARENA_ROPES, w.ropes)
+        self.upload(ARENA_META, w.meta)
+        self.upload(ARENA_MACRO, w.macro)
+        self.upload(ARENA_COLINFO, w.colinfo)
+        self.write_grid(w.grid)
+
+    def write_grid(self, grid: np.ndarray) -> None:
+        g = np.ascontiguousarray(grid, np.uint32)
+        _check(LIB.vx_write_grid(self._h, g.ctypes.data, len(g)), "vx_write_grid")
+
+    def write_clipmap(self, clip: Clipmap, heights: np.ndarray) -> None:
+        h = np.ascontiguousarray(heights, np.float32)
+        patches = (ClipPatch * max(1, clip.levels))()
+        for lv in range(clip.levels):
+            patches[lv] = ClipPatch(lv, 0, 0, clip.res, clip.res, h[lv].ctypes.data)
+        _check(LIB.vx_write_clipmap(self._h, C.byref(clip), patches, clip.levels), "vx_write_clipmap")
+
+    def sync(self) -> None:
+        _check(LIB.vx_sync(self._h), "vx_sync")
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class View:
+    """One camera stream with its own history textures."""
+
+    def __init__(self, ctx: Context, present_wh, render_wh=None):
+        render_wh = render_wh or present_wh
+        h = C.c_void_p()
+        _check(LIB.vx_view_create(ctx._h, present_wh[0], present_wh[1], render_wh[0], render_wh[1], C.byref(h)),
+               "vx_view_create")
+        self._h = h
+        self.ctx = ctx
+        self.pw, self.ph = present_wh
+        self.rw, self.rh = render_wh
+
+    def write_camera(self, cam: Camera) -> None:
+        _check(LIB.vx_write_camera(self._h, C.byref(cam)), "vx_write_camera")
+
+    def render(self, mask: int = PASS_FRAME, stream: int | None = None) -> None:
+        _check(LIB.vx_render(self._h, mask, stream), "vx_render")
+
+    def reset(self) -> None:
+        _check(LIB.vx_view_reset(self._h), "vx_view_reset")
+
+    def read(self, which: int) -> np.ndarray:
+        r, p = self.rw * self.rh, self.pw * self.ph
+        shapes = {
+            OUT_COLOR: ((self.rh, self.rw, 4), np.float32), OUT_DEPTH: ((self.rh, self.rw), np.float32),
+            OUT_LOCAL: ((self.rh, self.rw, 4), np.float32), OUT_PRIMARY_HIST: ((self.rh, self.rw, 4), np.float32),
+            OUT_SHADOW_HIST: ((self.rh, self.rw), np.float32), OUT_LOCAL_HIST: ((self.rh, self.rw, 4), np.float32),
+            OUT_GODRAY: ((self.rh, self.rw, 4), np.float32), OUT_PRE_TAA: ((self.ph, self.pw, 4), np.float32),
+            OUT_OUTPUT: ((self.ph, self.pw, 4), np.float32), OUT_HITS: ((self.rh, self.rw), HIT_DTYPE),
+        }
+        shape, dt = shapes[which]
+        out = np.empty(shape, dt)
+        _check(LIB.vx_read_output(self._h, which, out.ctypes.data, out.nbytes), "vx_read_output")
+        del r, p
+        return out
+
+    def write_history(self, which: int, data: np.ndarray) -> None:
+        d = np.ascontiguousarray(data)
+        _check(LIB.vx_write_history(self._h, which, d.ctypes.data, d.nbytes), "vx_write_history")
+
+    def counters(self, reset: bool = True) -> dict:
+        out = (C.c_uint64 * 9)()
+        _check(LIB.vx_view_counters(self._h, out, 1 if reset else 0), "vx_view_counters")
+        return dict(zip(COUNT_NAMES, (int(v) for v in out)))
+
+    def timings(self) -> list[float]:
+        ms = (C.c_float * 5)()
+        _check(LIB.vx_view_timings(self._h, ms), "vx_view_timings")
+        return [float(v) for v in ms]
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_view_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def algorithmic_bytes(counts: dict) -> int:
+    """Bytes of the reference's loads/stores from a counter dict (struct sizes)."""
+    return int(sum(counts[n] * b for n, b in zip(COUNT_NAMES, COUNT_BYTES)))
