@@ -19,7 +19,7 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libvoxhip.so"
+LIB_PATH = Path(os.environ.get("VOXHIP_LIB") or (Path(__file__).resolve().parent / "libvoxhip.so"))
 
 VX_OK = 0
 INVALID = 0xFFFFFFFF
@@ -216,17 +216,18 @@ class ChunkBuild:
     colinfo: np.ndarray
 
 
-def build_chunk(material: np.ndarray, ground: np.ndarray, chunk_oy: int, dirt_depth: int) -> ChunkBuild:
+def build_chunk(material: np.ndarray, ground: np.ndarray, chunk_oy: int, dirt_depth: int,
+                node_capacity: int = MAX_CHUNK_NODES) -> ChunkBuild:
     material = np.ascontiguousarray(material, np.uint8).reshape(-1)
     ground = np.ascontiguousarray(ground, np.int32).reshape(-1)
     assert material.size == 64 ** 3 and ground.size == 64 * 64
-    nodes = np.zeros(MAX_CHUNK_NODES, NODE_DTYPE)
-    ropes = np.zeros(MAX_CHUNK_NODES, ROPES_DTYPE)
+    nodes = np.zeros(node_capacity, NODE_DTYPE)
+    ropes = np.zeros(node_capacity, ROPES_DTYPE)
     macro = np.zeros(16, np.uint32)
     colinfo = np.zeros(2048, np.uint32)
     n = C.c_uint32(0)
     _check(LIB.vx_build_chunk(_ptr(material), _ptr(ground), chunk_oy, dirt_depth, _ptr(nodes), _ptr(ropes),
-                              MAX_CHUNK_NODES, C.byref(n), _ptr(macro), _ptr(colinfo)), "vx_build_chunk")
+                              node_capacity, C.byref(n), _ptr(macro), _ptr(colinfo)), "vx_build_chunk")
     return ChunkBuild(nodes[: n.value].copy(), ropes[: n.value].copy(), macro, colinfo)
 
 
