@@ -40,7 +40,38 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline)")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU sample (0: auto)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device three times (the plain copy kernel that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = REPO / "profiles" / "k_primary_traffic.md"
+
+
+def measured_traffic(W, H):
+    """HBM bytes per k_primary launch at this resolution, from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            val, _, res = line.split(":", 1)[1].partition("@")
+            if res.strip() == f"{W}x{H}":
+                return int(val.strip())
+    return None
+
+
+def copy_calibration(mib):
+    import torch
+
+    n = (mib << 20) // 4
+    src = torch.empty(n, dtype=torch.float32, device="cuda")
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def main():
@@ -115,7 +146,7 @@ def main():
         bytes_per_launch = vx.algorithmic_bytes(counts)
         achieved = bytes_per_launch / (prim_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(W, H),
                     "kernel": "k_primary", "algorithmic_bytes_per_launch": int(bytes_per_launch),
                     "algorithmic_bytes_per_ray": round(bytes_per_launch / rays, 1)}
 
@@ -151,6 +182,8 @@ def main():
     if counts is not None:
         out["primary_counts_per_launch"] = {k: int(v) for k, v in counts.items()}
     print(json.dumps(out), flush=True)
+    if a.copy_calib:
+        copy_calibration(a.copy_calib)
 
 
 def cpu_baseline(sc, hw, clip, heights, cam, W, H, rows):
