@@ -63,15 +63,22 @@ def measured_traffic(W, H):
 
 
 def copy_calibration(mib):
-    import torch
+    """Three device-to-device copies of MIB MiB through the HIP runtime's own
+    copy kernel: the reference workload for the FETCH_SIZE / WRITE_SIZE scale."""
+    import ctypes as C
 
-    n = (mib << 20) // 4
-    src = torch.empty(n, dtype=torch.float32, device="cuda")
-    src.fill_(1.0)
-    dst = torch.empty_like(src)
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(n))
     for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
+        hip.hipMemcpy(dst, src, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
 
 
 def main():

@@ -133,3 +133,39 @@ def test_godray_macro_binding_modes(gpu, fix):
     err = float(np.max(np.abs(view.read(gpu.OUT_GODRAY) - fr.godray)))
     print(f"[parity] godray fix={fix}: max_abs={err}", flush=True)
     assert err <= COLOR_ABS_TOL
+
+
+def test_flight_c1_world(gpu):
+    """The bench world (16x4x16 chunks, 5-level clipmap) along the seeded
+    camera flight, full pipeline, three frames at 640x360: the same world,
+    grass density and camera path as the 4K benchmark, at a size the oracle
+    renders in seconds."""
+    O = _oracle()
+    from voxhip import scenes
+
+    sc, hw = host_world("c1")
+    clip, heights = scenes.scene_clipmap(sc, 0)
+    res = (640, 360)
+    ctx = make_context(gpu, hw)
+    ctx.write_clipmap(clip, heights)
+    view = gpu.View(ctx, res, res)
+    w = O.World.from_host(hw, clip, heights)
+    fr = O.Frame(*res)
+    prev = None
+    for f in range(3):
+        frame = 30 + f
+        pose, t_s = sc.poses[frame]
+        cam = gpu.camera_build(pose, res, res, frame, time_s=t_s, wind_strength=2.0, grid_origin=sc.grid_origin,
+                               grid_dims=sc.grid_dims, chunk_count=hw.chunk_count, prev_view_proj=prev, jitter=True)
+        prev = np.array(cam.view_proj, np.float32)
+        view.write_camera(cam)
+        view.render(gpu.PASS_FRAME | gpu.PASS_WRITE_HITS)
+        O.frame(w, cam, fr, gpu.HIT_DTYPE)
+        s = summary(view.read(gpu.OUT_HITS), fr.hits, view.read(gpu.OUT_DEPTH), fr.depth, view.read(gpu.OUT_COLOR),
+                    fr.color)
+        s["output_max_abs"] = float(np.max(np.abs(view.read(gpu.OUT_OUTPUT) - fr.output)))
+        s["kinds"] = np.bincount(fr.hits["kind"].ravel(), minlength=3).tolist()
+        s["grass_hits"] = int(np.sum((fr.hits["kind"] == 1) & (fr.hits["mat"] == gpu.MAT_GRASS)))
+        _report(f"c1 flight frame {frame}", s)
+        _assert_parity(s)
+        assert s["output_max_abs"] <= COLOR_ABS_TOL

@@ -53,16 +53,21 @@ __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long
   }
 }
 
+// k_primary is latency bound (dependent node/rope loads): the register budget
+// is capped so that 6 waves fit per SIMD (<= 80 VGPRs) instead of 4.
 template <bool COUNT>
-__global__ __launch_bounds__(64) void k_primary(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb,
-                                                 uint32_t tiles_x, uint32_t ntiles, unsigned long long* __restrict__ counters) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_primary(
+    Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, uint32_t tiles_x, uint32_t ntiles,
+    unsigned long long* __restrict__ counters) {
   __shared__ TileLDS L;
   const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
   const uint32_t wgx = tile % tiles_x, wgy = tile / tiles_x;
+  if (threadIdx.x < 12u) L.grass_taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
   if constexpr (COUNT) {
     counted::Ctx C;
     C.S = S;
     C.cam = cam;
+    C.grass_taper = L.grass_taper;
     Counts cnt = {};
     C.cnt = &cnt;
     if (threadIdx.x == 0) counted::tile_build(C, L, wgx, wgy);
@@ -75,6 +80,7 @@ __global__ __launch_bounds__(64) void k_primary(Scene S, const vx_camera* __rest
     plain::Ctx C;
     C.S = S;
     C.cam = cam;
+    C.grass_taper = L.grass_taper;
     if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
     __syncthreads();
     const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
@@ -97,6 +103,7 @@ __global__ __launch_bounds__(64) void k_godray(Scene S, const vx_camera* __restr
     counted::Ctx C;
     C.S = S;
     C.cam = cam;
+    C.grass_taper = nullptr;  // the godray pass has no grass geometry
     Counts cnt = {};
     C.cnt = &cnt;
     if (gx < fb.rw && gy < fb.rh) counted::godray_pixel(C, fb, gx, gy);
@@ -105,6 +112,7 @@ __global__ __launch_bounds__(64) void k_godray(Scene S, const vx_camera* __restr
     plain::Ctx C;
     C.S = S;
     C.cam = cam;
+    C.grass_taper = nullptr;  // the godray pass has no grass geometry
     if (gx < fb.rw && gy < fb.rh) plain::godray_pixel(C, fb, gx, gy);
   }
 }
@@ -115,6 +123,7 @@ __global__ __launch_bounds__(256) void k_composite(Scene S, const vx_camera* __r
   plain::Ctx C;
   C.S = S;
   C.cam = cam;
+  C.grass_taper = nullptr;
   fb.pre_taa[(size_t)y * fb.pw + x] = plain::composite_pixel(C, fb, x, y);
 }
 
@@ -124,6 +133,7 @@ __global__ __launch_bounds__(256) void k_composite_taa(Scene S, const vx_camera*
   plain::Ctx C;
   C.S = S;
   C.cam = cam;
+  C.grass_taper = nullptr;
   plain::composite_taa_pixel(C, fb, x, y);
 }
 

@@ -59,6 +59,7 @@ struct TileLDS {
   uint32_t cached;
   uint32_t slots[kMaxTileChunks];
   float enter[kMaxTileChunks];
+  float grass_taper[12];  // blade taper per (lod, segment), see grass_taper_entry
 };
 
 }  // namespace vxd
