@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--res", default="3840x2160")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--staged", action="store_true", help="primary pass as staged kernels over pixel queues")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline)")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU sample (0: auto)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -96,7 +97,7 @@ def main():
     hw = vx.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
     clip, heights = scenes.scene_clipmap(sc, 0)
     ctx = vx.Context(node_capacity=len(hw.nodes), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
-                     clip_levels=5, clip_res=256)
+                     clip_levels=5, clip_res=256, staged=a.staged)
     ctx.upload_world(hw)
     ctx.write_clipmap(clip, heights)
     view = vx.View(ctx, (W, H), (W, H))
@@ -177,7 +178,7 @@ def main():
         "config": {"workload": f"plain one-view-per-launch full pipeline at {W}x{H}, 16x4x16-chunk world, "
                                f"5-level clipmap, camera flight frames {a.warmup}..{a.warmup + a.steps - 1}",
                    "resolution": [W, H], "chunks": hw.chunk_count, "nodes": int(len(hw.nodes)),
-                   "path": "plain"},
+                   "path": "plain", "primary_kernels": "staged" if a.staged else "one per tile"},
         "primary_kernel_rays_per_s": round(rays / (prim_ms * 1e-3), 1),
         "pass_ms_mean": {n: round(float(v), 4) for n, v in
                          zip(("primary", "local_taa", "godray", "composite", "composite_taa"), pass_ms.mean(axis=0))},

@@ -1,0 +1,25 @@
+// Host stand-in for the few HIP names the device pass code uses, so that
+// voxel-engine-v0_amd/csrc/vxd_passes.inc compiles as ordinary C++ for a CPU
+// test build.  Test infrastructure only: the product is built by hipcc.
+#pragma once
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __restrict__ __restrict
+
+struct float4 { float x, y, z, w; };
+struct uint4 { uint32_t x, y, z, w; };
+inline float4 make_float4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
+inline uint4 make_uint4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return uint4{x, y, z, w}; }
+inline uint32_t __float_as_uint(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+inline float __uint_as_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+inline int __clz(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+inline int __popc(uint32_t x) { return __builtin_popcount(x); }
+inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
+inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
+inline int32_t min(int32_t a, int32_t b) { return a < b ? a : b; }
+inline int32_t max(int32_t a, int32_t b) { return a > b ? a : b; }

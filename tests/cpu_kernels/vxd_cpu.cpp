@@ -1,0 +1,245 @@
+// vxd_cpu.cpp — the product's device pass code (vxd_passes.inc) compiled for
+// the host, with host loops standing in for the kernels' tile, lane and queue
+// structure.  Test infrastructure only: it lets the CPU test suite check the
+// exact code the GPU runs (results, load counts, loop termination) against
+// the oracle without a GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <thread>
+#include <vector>
+
+#include "vxd_consts.h"
+#include "vxd_scene.h"
+
+namespace vxd {
+namespace plain {
+#define VXD_COUNT 0
+#include "vxd_passes.inc"
+#undef VXD_COUNT
+}  // namespace plain
+namespace counted {
+#define VXD_COUNT 1
+#include "vxd_passes.inc"
+#undef VXD_COUNT
+}  // namespace counted
+}  // namespace vxd
+
+using namespace vxd;
+
+extern "C" {
+
+// Same layout as the oracle's scene description.
+typedef struct cpu_scene {
+  const vx_chunk_meta* chunks;
+  uint64_t n_chunks;
+  const vx_node* nodes;
+  const vx_node_ropes* ropes;
+  uint64_t n_nodes;
+  const uint32_t* grid;
+  uint64_t n_grid;
+  const uint32_t* macro_words;
+  uint64_t n_macro;
+  const uint32_t* colinfo;
+  uint64_t n_colinfo;
+  const vx_clipmap* clip;
+  const float* clip_height;
+  uint64_t n_clip;
+} cpu_scene;
+
+}  // extern "C"
+
+namespace {
+
+Scene to_scene(const cpu_scene* s) {
+  Scene S;
+  S.chunks = s->chunks;
+  S.nodes = reinterpret_cast<const uint4*>(s->nodes);
+  S.ropes = reinterpret_cast<const uint4*>(s->ropes);
+  S.grid = s->grid;
+  S.macro = s->macro_words;
+  S.colinfo = s->colinfo;
+  S.clip = s->clip;
+  S.clip_height = s->clip_height;
+  S.n_chunks = s->n_chunks;
+  S.n_nodes = s->n_nodes;
+  S.n_grid = s->n_grid;
+  S.n_macro = s->n_macro;
+  S.n_colinfo = s->n_colinfo;
+  S.n_clip = s->n_clip;
+  return S;
+}
+
+template <class F>
+void parallel_for(uint32_t n, int threads, F f) {
+  if (threads <= 1 || n < 2) {
+    for (uint32_t i = 0; i < n; ++i) f(i, 0);
+    return;
+  }
+  std::atomic<uint32_t> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&, t] {
+      for (uint32_t i; (i = next.fetch_add(1)) < n;) f(i, t);
+    });
+  for (auto& th : pool) th.join();
+}
+
+void fill_taper(float* taper) {
+  for (uint32_t j = 0; j < 12u; ++j) taper[j] = plain::grass_taper_entry(j);
+}
+
+}  // namespace
+
+extern "C" {
+
+// mode 0: the one-kernel pass (k_primary<false>); 1: the staged kernels over
+// pixel queues; 2: the load-counting kernel (k_primary<true>), which also adds
+// its counts into counts[9].
+int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, uint32_t h, const float* primary_hist_in,
+                    const float* shadow_in, float* color, float* depth, float* local, float* primary_hist_out,
+                    float* shadow_out, vx_hit* hits, int mode, int threads, uint64_t* counts) {
+  if (!scene || !cam || !w || !h) return -1;
+  const Scene S = to_scene(scene);
+  FrameBuffers fb{};
+  fb.rw = fb.pw = w;
+  fb.rh = fb.ph = h;
+  fb.primary_hist_in = reinterpret_cast<const float4*>(primary_hist_in);
+  fb.primary_hist_out = reinterpret_cast<float4*>(primary_hist_out);
+  fb.shadow_in = shadow_in;
+  fb.shadow_out = shadow_out;
+  fb.color = reinterpret_cast<float4*>(color);
+  fb.depth = depth;
+  fb.local = reinterpret_cast<float4*>(local);
+  fb.hits = hits;
+  const uint32_t tx = (w + 7u) / 8u, ty = (h + 7u) / 8u, nt = tx * ty;
+  threads = std::max(1, threads);
+  std::vector<Counts> cnt((size_t)threads, Counts{});
+
+  if (mode == 0 || mode == 2) {
+    parallel_for(nt, threads, [&](uint32_t tile, int t) {
+      TileLDS L;
+      fill_taper(L.grass_taper);
+      const uint32_t wgx = tile % tx, wgy = tile / tx;
+      if (mode == 0) {
+        plain::Ctx C;
+        C.S = S;
+        C.cam = cam;
+        C.grass_taper = L.grass_taper;
+        plain::tile_build(C, L, wgx, wgy);
+        for (uint32_t lane = 0; lane < 64u; ++lane) {
+          const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
+          if (gx < w && gy < h) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+        }
+      } else {
+        counted::Ctx C;
+        C.S = S;
+        C.cam = cam;
+        C.grass_taper = L.grass_taper;
+        C.cnt = &cnt[t];
+        counted::tile_build(C, L, wgx, wgy);
+        for (uint32_t lane = 0; lane < 64u; ++lane) {
+          const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
+          if (gx < w && gy < h) counted::primary_pixel(C, L, fb, gx, gy, counted::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+        }
+      }
+    });
+  } else if (mode == 1) {
+    // Staged: trace per tile into per-class queues, then each stage over its queue.
+    const size_t n = (size_t)w * h;
+    std::vector<PrimRec> rec(n);
+    std::vector<std::vector<uint32_t>> tq[kNumPrimQueues];
+    for (auto& q : tq) q.resize(nt);
+    float taper[12];
+    fill_taper(taper);
+    parallel_for(nt, threads, [&](uint32_t tile, int) {
+      TileLDS L;
+      fill_taper(L.grass_taper);
+      const uint32_t wgx = tile % tx, wgy = tile / tx;
+      plain::Ctx C;
+      C.S = S;
+      C.cam = cam;
+      C.grass_taper = L.grass_taper;
+      plain::tile_build(C, L, wgx, wgy);
+      for (uint32_t lane = 0; lane < 64u; ++lane) {
+        const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
+        if (gx >= w || gy >= h) continue;
+        const plain::PrimRay r = plain::prim_ray(C, fb, gx, gy);
+        const plain::VoxTraceResult vt = plain::prim_trace(C, L, fb, r);
+        const uint32_t pix = (uint32_t)r.pix;
+        if (plain::prim_is_voxel(vt)) {
+          tq[kQVox][tile].push_back(pix);
+          if (vt.best.mat == MAT_GRASS) tq[kQGrass][tile].push_back(pix);
+          if (plain::prim_shadow_pixel(r)) tq[kQShadow][tile].push_back(pix);
+          plain::prim_rec_store(rec.data(), pix, vt.best, plain::prim_anchor(vt));
+        } else {
+          tq[kQEnv][tile].push_back(pix);
+        }
+      }
+    });
+    std::vector<uint32_t> q[kNumPrimQueues];
+    for (uint32_t k = 0; k < kNumPrimQueues; ++k)
+      for (auto& part : tq[k]) q[k].insert(q[k].end(), part.begin(), part.end());
+    auto ctx = [&]() {
+      plain::Ctx C;
+      C.S = S;
+      C.cam = cam;
+      C.grass_taper = taper;
+      return C;
+    };
+    auto ray = [&](const plain::Ctx& C, uint32_t pix) { return plain::prim_ray(C, fb, pix % w, pix / w); };
+    parallel_for((uint32_t)q[kQGrass].size(), threads, [&](uint32_t i, int) {
+      const plain::Ctx C = ctx();
+      const uint32_t pix = q[kQGrass][i];
+      plain::PrimAnchor a;
+      plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
+      plain::prim_grass_refine(C, ray(C, pix), hg);
+      plain::prim_rec_store_tn(rec.data(), pix, hg);
+    });
+    parallel_for((uint32_t)q[kQShadow].size(), threads, [&](uint32_t i, int) {
+      const plain::Ctx C = ctx();
+      const uint32_t pix = q[kQShadow][i];
+      plain::PrimAnchor a;
+      const plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
+      fb.shadow_out[pix] = plain::prim_shadow(C, fb, ray(C, pix), hg);
+    });
+    std::vector<uint8_t> lit(n, 0);
+    parallel_for((uint32_t)q[kQVox].size(), threads, [&](uint32_t i, int) {
+      const plain::Ctx C = ctx();
+      const uint32_t pix = q[kQVox][i];
+      const plain::PrimRay r = ray(C, pix);
+      plain::PrimAnchor a;
+      const plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
+      const float shadow_store = plain::prim_shadow_pixel(r) ? fb.shadow_out[pix] : plain::prim_shadow(C, fb, r, hg);
+      lit[pix] = plain::prim_shade_voxel(C, fb, r, hg, shadow_store, a, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+      if (!lit[pix]) fb.local[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    });
+    for (uint32_t pix : q[kQVox])
+      if (lit[pix]) q[kQLight].push_back(pix);
+    parallel_for((uint32_t)q[kQLight].size(), threads, [&](uint32_t i, int) {
+      const plain::Ctx C = ctx();
+      const uint32_t pix = q[kQLight][i];
+      plain::PrimAnchor a;
+      const plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
+      fb.local[pix] = plain::prim_local_light(C, ray(C, pix), hg);
+    });
+    parallel_for((uint32_t)q[kQEnv].size(), threads, [&](uint32_t i, int) {
+      const plain::Ctx C = ctx();
+      const uint32_t pix = q[kQEnv][i];
+      plain::prim_env(C, fb, ray(C, pix), plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+    });
+  } else {
+    return -1;
+  }
+  if (counts) {
+    for (const Counts& c : cnt) {
+      const uint32_t v[kNumCounts] = {c.grid, c.meta, c.macro, c.node, c.rope, c.colinfo, c.clip, c.tex_rd, c.tex_wr};
+      for (int i = 0; i < kNumCounts; ++i) counts[i] += v[i];
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

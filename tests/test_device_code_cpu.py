@@ -1,0 +1,54 @@
+"""The product's device pass code (voxel-engine-v0_amd/csrc/vxd_passes.inc),
+compiled for the host by tests/cpu_kernels, against the oracle: the primary
+pass in its one-kernel form, its staged form (the stage functions run over
+per-class pixel queues) and its load-counting form.  Hit identity, colour,
+depth, history and shadow must be bit-identical, and the counting form must
+count exactly the loads the oracle counts.  Any loop that failed to end would
+hang here, on the CPU, before the code ever runs on a GPU."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import vxd_cpu
+from conftest import host_world
+from parity import summary
+
+
+def _frames(name, res, frame_ids):
+    import voxhip as vx
+    from oracle import oracle as O
+    from voxhip import scenes
+
+    sc, hw = host_world(name)
+    clip, heights = scenes.scene_clipmap(sc, 0)
+    world = O.World.from_host(hw, clip, heights)
+    prev = None
+    for f in frame_ids:
+        pose, t_s = sc.poses[f if f < len(sc.poses) else 0]
+        cam = vx.camera_build(pose, res, res, f, time_s=t_s, wind_strength=2.0, grid_origin=sc.grid_origin,
+                              grid_dims=sc.grid_dims, chunk_count=hw.chunk_count, prev_view_proj=prev, jitter=True)
+        prev = np.array(cam.view_proj, np.float32)
+        yield world, cam
+
+
+@pytest.mark.parametrize("name,res,frame_ids", [("c0", (320, 180), [0, 1]), ("c1", (320, 180), [30, 31, 32])])
+def test_device_code_matches_oracle(vx, name, res, frame_ids):
+    from oracle import oracle as O
+
+    ref = O.Frame(*res)
+    frs = {m: O.Frame(*res) for m in vxd_cpu.MODES}
+    for world, cam in _frames(name, res, frame_ids):
+        oc = O.Counts()
+        O.primary(world, cam, ref, vx.HIT_DTYPE, counts=oc)
+        for mode, fr in frs.items():
+            counts = vxd_cpu.primary(world, cam, fr, vx.HIT_DTYPE, mode=mode)
+            s = summary(fr.hits, ref.hits, fr.depth, ref.depth, fr.color, ref.color)
+            assert s["identity_mismatches"] == 0 and s["t_bit_exact"] and s["color_bit_exact"], (mode, s)
+            assert np.array_equal(fr.depth.view(np.uint32), ref.depth.view(np.uint32)), mode
+            assert np.array_equal(fr.primary_hist.view(np.uint32), ref.primary_hist.view(np.uint32)), mode
+            assert np.array_equal(fr.shadow.view(np.uint32), ref.shadow.view(np.uint32)), mode
+            assert np.array_equal(fr.local.view(np.uint32), ref.local.view(np.uint32)), mode
+            if mode == "counted":
+                assert [int(c) for c in counts] == [oc.as_dict()[n] for n in vx.COUNT_NAMES]
+        assert int(np.sum(ref.hits["kind"] == 1)) > 0

@@ -1,0 +1,64 @@
+"""The product's device pass code built for the host (tests/cpu_kernels):
+builds libvxd_cpu.so with g++ on first use and runs the primary pass in its
+one-kernel, staged and load-counting forms with host loops in place of the
+GPU's tiles, lanes and queues."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parent
+SRC = HERE / "cpu_kernels" / "vxd_cpu.cpp"
+OUT = HERE / "cpu_kernels" / "_build" / "libvxd_cpu.so"
+CSRC = REPO / "voxel-engine-v0_amd" / "csrc"
+DEPS = [SRC, CSRC / "vxd_passes.inc", CSRC / "vxd_math.h", CSRC / "vxd_consts.h", CSRC / "vxd_scene.h",
+        REPO / "include" / "voxhip.h", HERE / "cpu_kernels" / "shim" / "hip" / "hip_runtime.h"]
+MODES = {"one": 0, "staged": 1, "counted": 2}
+
+
+def build() -> Path:
+    if OUT.exists() and all(OUT.stat().st_mtime >= d.stat().st_mtime for d in DEPS):
+        return OUT
+    OUT.parent.mkdir(parents=True, exist_ok=True)
+    tmp = OUT.with_suffix(".so.tmp")
+    cmd = ["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-w",
+           f"-I{HERE / 'cpu_kernels' / 'shim'}", f"-I{REPO / 'include'}", f"-I{CSRC}", "-o", str(tmp), str(SRC),
+           "-lpthread"]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = C.CDLL(str(build()))
+        vp = C.c_void_p
+        _LIB.vxd_cpu_primary.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,
+                                         vp]
+        _LIB.vxd_cpu_primary.restype = C.c_int
+    return _LIB
+
+
+def primary(world, cam, fr, hits_dtype, mode: str = "one", threads: int = 0):
+    """Same contract as oracle.primary: updates fr in place; returns the load counts for mode 'counted'."""
+    threads = threads or min(16, os.cpu_count() or 1)
+    hist_out = np.zeros_like(fr.primary_hist)
+    shadow_out = np.zeros_like(fr.shadow)
+    hits = np.zeros((fr.rh, fr.rw), hits_dtype)
+    counts = np.zeros(9, np.uint64)
+    rc = lib().vxd_cpu_primary(C.addressof(world.scene), C.addressof(cam), fr.rw, fr.rh, fr.primary_hist.ctypes.data,
+                               fr.shadow.ctypes.data, fr.color.ctypes.data, fr.depth.ctypes.data, fr.local.ctypes.data,
+                               hist_out.ctypes.data, shadow_out.ctypes.data, hits.ctypes.data, MODES[mode], threads,
+                               counts.ctypes.data)
+    assert rc == 0
+    fr.primary_hist, fr.shadow, fr.hits = hist_out, shadow_out, hits
+    return counts

@@ -5,8 +5,13 @@
 
 #include "vxd_scene.h"
 
+// The primary pass as one kernel per 8x8 tile (count: the load-counting form).
 hipError_t vx_launch_primary(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
                              unsigned long long* counters, hipStream_t st);
+// The primary pass as staged kernels (trace, grass refinement, shadow, voxel
+// shading, light gather, heightfield/sky) over pixel queues.
+hipError_t vx_launch_primary_staged(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb,
+                                    const vxd::PrimQueues& Q, hipStream_t st);
 hipError_t vx_launch_local_taa(const vxd::FrameBuffers& fb, hipStream_t st);
 hipError_t vx_launch_godray(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
                             unsigned long long* counters, hipStream_t st);

@@ -65,6 +65,10 @@ struct vx_view {
   float4* output = nullptr;
   float4* out_hist[2] = {nullptr, nullptr};
   vx_hit* hits = nullptr;
+  // Primary-pass stage buffers: voxel-hit records, pixel queues, counters.
+  vxd::PrimRec* prim_rec = nullptr;
+  uint32_t* prim_q[vxd::kNumPrimQueues] = {};
+  uint32_t* prim_count = nullptr;
   unsigned long long* counters = nullptr;
   // Slot written last by each history-producing pass.
   int written_primary = 1, written_local = 1, written_godray = 1, written_out = 1;
@@ -137,6 +141,9 @@ void free_view_buffers(vx_view* v) {
   dfree(v->pre_taa);
   dfree(v->output);
   dfree(v->hits);
+  dfree(v->prim_rec);
+  for (auto& q : v->prim_q) dfree(q);
+  dfree(v->prim_count);
 }
 
 int alloc_view_buffers(vx_view* v) {
@@ -149,6 +156,9 @@ int alloc_view_buffers(vx_view* v) {
       return rc;
   }
   if ((rc = dalloc(&v->pre_taa, p)) || (rc = dalloc(&v->output, p)) || (rc = dalloc(&v->hits, r))) return rc;
+  if ((rc = dalloc(&v->prim_rec, r)) || (rc = dalloc(&v->prim_count, vxd::kNumPrimQueues))) return rc;
+  for (auto& q : v->prim_q)
+    if ((rc = dalloc(&q, r))) return rc;
   v->ping = 0;
   v->written_primary = v->written_local = v->written_godray = v->written_out = 1;
   return VX_OK;
@@ -401,7 +411,15 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
   const vxd::Scene S = make_scene(c, false);
   VX_HIP(hipEventRecord(v->ev[0], st));
   if (mask & VX_PASS_PRIMARY) {
-    VX_HIP(vx_launch_primary(S, v->cam_dev, fb, count, v->counters, st));
+    if (count || !(c->flags & VX_CTX_PRIMARY_STAGED)) {
+      VX_HIP(vx_launch_primary(S, v->cam_dev, fb, count, v->counters, st));
+    } else {
+      vxd::PrimQueues Q;
+      Q.rec = v->prim_rec;
+      for (uint32_t i = 0; i < vxd::kNumPrimQueues; ++i) Q.q[i] = v->prim_q[i];
+      Q.count = v->prim_count;
+      VX_HIP(vx_launch_primary_staged(S, v->cam_dev, fb, Q, st));
+    }
     v->written_primary = pong;
   }
   VX_HIP(hipEventRecord(v->ev[1], st));

@@ -41,6 +41,7 @@ OUT_LOCAL_HIST, OUT_GODRAY, OUT_PRE_TAA, OUT_OUTPUT, OUT_HITS = 5, 6, 7, 8, 9
 
 ARENA_NODES, ARENA_ROPES, ARENA_META, ARENA_MACRO, ARENA_COLINFO = range(5)
 CTX_GODRAY_MACRO_FIX = 1
+CTX_PRIMARY_STAGED = 2
 CAM_JITTER = 1
 
 NODE_DTYPE = np.dtype([("child_base", "<u4"), ("child_mask", "<u4"), ("material", "<u4"), ("key", "<u4")])
@@ -325,9 +326,9 @@ class Context:
     """Resident world on one GPU (Renderer::new + uploads)."""
 
     def __init__(self, node_capacity: int, chunk_capacity: int, grid_capacity: int, clip_levels: int = 5,
-                 clip_res: int = 256, device: int = 0, godray_macro_fix: bool = False):
-        caps = Caps(node_capacity, chunk_capacity, grid_capacity, clip_levels, clip_res, device,
-                    CTX_GODRAY_MACRO_FIX if godray_macro_fix else 0)
+                 clip_res: int = 256, device: int = 0, godray_macro_fix: bool = False, staged: bool = False):
+        flags = (CTX_GODRAY_MACRO_FIX if godray_macro_fix else 0) | (CTX_PRIMARY_STAGED if staged else 0)
+        caps = Caps(node_capacity, chunk_capacity, grid_capacity, clip_levels, clip_res, device, flags)
         h = C.c_void_p()
         _check(LIB.vx_create(C.byref(caps), C.byref(h)), "vx_create")
         self._h = h
