@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--res", default="3840x2160")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--staged", action="store_true", help="primary pass as staged kernels over pixel queues")
+    ap.add_argument("--batch-views", type=int, default=64,
+                    help="views of the mixed-resolution batching workload (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-1080p", action="store_true", help="skip the 1080p per-view latency run")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline)")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU sample (0: auto)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -158,6 +162,9 @@ def main():
                     "kernel": "k_primary", "algorithmic_bytes_per_launch": int(bytes_per_launch),
                     "algorithmic_bytes_per_ray": round(bytes_per_launch / rays, 1)}
 
+    lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup)
+    batched = None if a.batch_views <= 0 else batching(vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(sc, hw, clip, heights, cams[a.warmup], W, H, a.cpu_rows)
@@ -185,6 +192,8 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "latency_1080p_ms": lat1080,
+        "batched": batched,
         "setup_s": round(setup_s, 2),
     }
     if counts is not None:
@@ -192,6 +201,84 @@ def main():
     print(json.dumps(out), flush=True)
     if a.copy_calib:
         copy_calibration(a.copy_calib)
+
+
+def latency_1080p(vx, scenes, sc, hw, ctx, steps, warmup):
+    """Per-view frame latency of one 1920x1080 view along the same flight
+    (camera write -> all five passes done), p50/p99 in ms."""
+    res = (1920, 1080)
+    view = vx.View(ctx, res, res)
+    lat, prev = [], None
+    for f in range(warmup + steps):
+        cam = scenes.frame_camera(sc, hw, f, prev_view_proj=prev, jitter=True, render=res)
+        prev = np.array(cam.view_proj, np.float32)
+        t = time.perf_counter()
+        view.write_camera(cam)
+        view.render(vx.PASS_FRAME)
+        view.timings()
+        if f >= warmup:
+            lat.append((time.perf_counter() - t) * 1e3)
+    view.close()
+    return {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+            "frames": steps}
+
+
+def batching(vx, scenes, sc, hw, ctx, n_views, steps):
+    """The mixed-resolution multi-view workload: n_views views with seeded
+    resolutions from {320x180 .. 3840x2160} and seeded poses over the same
+    world, each step one full frame of every view.  Plain = one view per
+    launch sequence (vx_render per view); batched = one fused launch per pass
+    (vx_batch_render), after the autotuner picked its configuration."""
+    rng = np.random.default_rng(7)
+    sizes = [(320, 180), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
+    res = [sizes[i] for i in rng.integers(0, len(sizes), n_views)]
+    frames = rng.integers(0, len(sc.poses), n_views)
+    rays = int(sum(w * h for w, h in res))
+
+    def cams(step):
+        return [scenes.frame_camera(sc, hw, int(frames[i]) + step, jitter=True, render=res[i]) for i in range(n_views)]
+
+    def run(views, render_step):
+        times = []
+        for step in range(steps + 1):
+            cs = cams(step)
+            ctx.sync()
+            t = time.perf_counter()
+            render_step(views, cs)
+            ctx.sync()
+            if step > 0:  # step 0 warms up
+                times.append(time.perf_counter() - t)
+        return float(np.mean(times))
+
+    def plain_step(views, cs):
+        for v, c in zip(views, cs):
+            v.write_camera(c)
+            v.render(vx.PASS_FRAME)
+
+    views = [vx.View(ctx, r, r) for r in res]
+    t_plain = run(views, plain_step)
+    for v in views:
+        v.close()
+    views = [vx.View(ctx, r, r) for r in res]
+    batch = vx.Batch(ctx, views)
+    for v, c in zip(views, cams(0)):
+        v.write_camera(c)
+    tuned = batch.tune(vx.PASS_FRAME, iters=1).as_dict()
+
+    def batched_step(vs, cs):
+        for v, c in zip(vs, cs):
+            v.write_camera(c)
+        batch.render(vx.PASS_FRAME)
+
+    t_batch = run(views, batched_step)
+    batch.close()
+    for v in views:
+        v.close()
+    return {"workload": f"{n_views} views, resolutions uniform over 320x180..3840x2160 (seed 7), full pipeline, "
+                        f"same world and clipmap", "rays_per_step": rays,
+            "plain_rays_per_s": round(rays / t_plain, 1), "batched_rays_per_s": round(rays / t_batch, 1),
+            "speedup": round(t_plain / t_batch, 3), "plain_ms_per_step": round(t_plain * 1e3, 2),
+            "batched_ms_per_step": round(t_batch * 1e3, 2), "tuned": tuned}
 
 
 def cpu_baseline(sc, hw, clip, heights, cam, W, H, rows):

@@ -268,6 +268,36 @@ typedef struct vx_ray {
 int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev,
                   uint32_t n, vx_hit* hits_dev, void* stream);
 
+/* Batches: many views of one context rendered together (the serving and
+ * batching path; no reference counterpart: the reference renders one view per
+ * encode_compute).  Each pass runs as ONE launch over all views: primary and
+ * godray as persistent work-stealing kernels over the union of the views' 8x8
+ * tiles (largest views first, one tile band per XCD, stealing across bands),
+ * the streaming passes as one flattened grid.  Results are identical to
+ * rendering each view with vx_render.  Each view keeps its own camera
+ * (vx_write_camera) and history; VX_PASS_END_FRAME flips every view's history.
+ * A batch refers to its views' buffers: resizing a view invalidates the batch
+ * (vx_batch_render then fails with VX_E_INVALID). */
+typedef struct vx_batch vx_batch;
+typedef struct vx_batch_config {
+    uint32_t waves_per_simd;  /* persistent workgroups = CUs * 4 SIMDs * this (default 6) */
+    uint32_t grab;            /* tiles taken per work-stealing atomic (default 1) */
+    uint32_t use_graph;       /* replay the pass sequence from a captured hipGraph (default 1) */
+    uint32_t pad;
+} vx_batch_config;
+int  vx_batch_create(vx_ctx* ctx, vx_view* const* views, uint32_t n, vx_batch** out);
+void vx_batch_destroy(vx_batch* batch);
+/* pass_mask as for vx_render, without VX_PASS_WRITE_HITS / VX_PASS_COUNT. */
+int  vx_batch_render(vx_batch* batch, uint32_t pass_mask, void* stream);
+/* GPU milliseconds of the last vx_batch_render (waits for it). */
+int  vx_batch_timing(vx_batch* batch, float* ms);
+int  vx_batch_set_config(vx_batch* batch, const vx_batch_config* cfg);
+int  vx_batch_get_config(const vx_batch* batch, vx_batch_config* cfg);
+/* Autotuner: times the passes in pass_mask (history is not advanced) for
+ * waves_per_simd in {4, 6, 8} x grab in {1, 2}, `iters` runs each,
+ * keeps the fastest configuration and returns it in *chosen (may be NULL). */
+int  vx_batch_tune(vx_batch* batch, uint32_t pass_mask, uint32_t iters, vx_batch_config* chosen);
+
 /* Blocks until all work queued on the context's stream is done. */
 int vx_sync(vx_ctx* ctx);
 

@@ -294,6 +294,137 @@ __global__ __launch_bounds__(256) void k_composite_taa(Scene S, const vx_camera*
   plain::composite_taa_pixel(C, fb, x, y);
 }
 
+// ---------------------------------------------------------------------------
+// Fused multi-view launches (a batch of views rendered together).
+//
+// Primary and godray run as persistent work-stealing kernels over the union
+// of the views' 8x8 tiles (views in table order, largest first).  The tile
+// space is cut into 8 contiguous bands, one per XCD; a workgroup takes tiles
+// from its own XCD's band first (L2 locality), then steals from the other
+// bands in turn.  Every workgroup exits once all 8 bands are exhausted.  The
+// streaming passes run one flattened grid over all views' pixel blocks.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t batch_view_of(const Batch& B, uint32_t item, int which) {
+  uint32_t lo = 0u, hi = B.n - 1u;  // last view whose base <= item
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1u) >> 1;
+    const BatchView& V = B.views[mid];
+    const uint32_t base = which == 0 ? V.tile_base : (which == 1 ? V.rblk_base : V.pblk_base);
+    if (base <= item) lo = mid;
+    else hi = mid - 1u;
+  }
+  return lo;
+}
+
+// Next run of up to B.grab tiles for this workgroup, or false when every band
+// is exhausted.  `band` is the band this workgroup is currently draining; it
+// starts at the workgroup's XCD and advances when the band runs dry.
+__device__ __forceinline__ bool batch_take(const Batch& B, uint32_t pass, uint32_t& band, uint32_t& steps, uint32_t& t0,
+                                           uint32_t& t1) {
+  const uint32_t per = (B.tiles + kBatchBands - 1u) / kBatchBands;
+  while (steps < kBatchBands) {
+    const uint32_t lo = band * per, hi = min(B.tiles, lo + per);
+    uint32_t got = 0u;
+    if (threadIdx.x == 0) got = atomicAdd(B.cursor + pass * kBatchBands + band, B.grab);
+    got = __shfl(got, 0, 64);
+    if (lo + got < hi) {
+      t0 = lo + got;
+      t1 = min(hi, t0 + B.grab);
+      return true;
+    }
+    band = (band + 1u) % kBatchBands;
+    ++steps;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(64) void k_batch_begin(Batch B) {
+  if (threadIdx.x < 2u * kBatchBands) B.cursor[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(64) void k_batch_end(Batch B) {
+  for (uint32_t v = threadIdx.x; v < B.n; v += 64u) B.ping[v] ^= 1u;
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_primary_batch(Scene S, Batch B) {
+  __shared__ TileLDS L;
+  if (threadIdx.x < 12u) L.grass_taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
+  uint32_t band = blockIdx.x % kBatchBands, steps = 0u, t0 = 0u, t1 = 0u;
+  while (batch_take(B, 0u, band, steps, t0, t1)) {
+    for (uint32_t t = t0; t < t1; ++t) {
+      const uint32_t v = batch_view_of(B, t, 0);
+      const BatchView& V = B.views[v];
+      const FrameBuffers& fb = V.fb[B.ping[v] & 1u];
+      const uint32_t tile = t - V.tile_base;
+      const uint32_t wgx = tile % V.tiles_x, wgy = tile / V.tiles_x;
+      plain::Ctx C;
+      C.S = S;
+      C.cam = V.cam;
+      C.grass_taper = L.grass_taper;
+      __syncthreads();  // the previous tile's lanes are done with L
+      if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
+      __syncthreads();
+      const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
+      if (gx < fb.rw && gy < fb.rh) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void k_godray_batch(Scene S, Batch B) {
+  uint32_t band = blockIdx.x % kBatchBands, steps = 0u, t0 = 0u, t1 = 0u;
+  while (batch_take(B, 1u, band, steps, t0, t1)) {
+    for (uint32_t t = t0; t < t1; ++t) {
+      const uint32_t v = batch_view_of(B, t, 0);
+      const BatchView& V = B.views[v];
+      const FrameBuffers& fb = V.fb[B.ping[v] & 1u];
+      const uint32_t tile = t - V.tile_base;
+      const uint32_t gx = (tile % V.tiles_x) * 8u + (threadIdx.x & 7u), gy = (tile / V.tiles_x) * 8u + (threadIdx.x >> 3);
+      plain::Ctx C;
+      C.S = S;
+      C.cam = V.cam;
+      C.grass_taper = nullptr;
+      if (gx < fb.rw && gy < fb.rh) plain::godray_pixel(C, fb, gx, gy);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_local_taa_batch(Batch B) {
+  const uint32_t v = batch_view_of(B, blockIdx.x, 1);
+  const BatchView& V = B.views[v];
+  const FrameBuffers& fb = V.fb[B.ping[v] & 1u];
+  const uint32_t blk = blockIdx.x - V.rblk_base;
+  const uint32_t x = (blk % V.rblk_x) * 16u + (threadIdx.x & 15u), y = (blk / V.rblk_x) * 16u + (threadIdx.x >> 4);
+  if (x < fb.rw && y < fb.rh) plain::local_taa_pixel(fb, x, y);
+}
+
+__global__ __launch_bounds__(256) void k_composite_batch(Scene S, Batch B) {
+  const uint32_t v = batch_view_of(B, blockIdx.x, 2);
+  const BatchView& V = B.views[v];
+  const FrameBuffers& fb = V.fb[B.ping[v] & 1u];
+  const uint32_t blk = blockIdx.x - V.pblk_base;
+  const uint32_t x = (blk % V.pblk_x) * 16u + (threadIdx.x & 15u), y = (blk / V.pblk_x) * 16u + (threadIdx.x >> 4);
+  if (x >= fb.pw || y >= fb.ph) return;
+  plain::Ctx C;
+  C.S = S;
+  C.cam = V.cam;
+  C.grass_taper = nullptr;
+  fb.pre_taa[(size_t)y * fb.pw + x] = plain::composite_pixel(C, fb, x, y);
+}
+
+__global__ __launch_bounds__(256) void k_composite_taa_batch(Scene S, Batch B) {
+  const uint32_t v = batch_view_of(B, blockIdx.x, 2);
+  const BatchView& V = B.views[v];
+  const FrameBuffers& fb = V.fb[B.ping[v] & 1u];
+  const uint32_t blk = blockIdx.x - V.pblk_base;
+  const uint32_t x = (blk % V.pblk_x) * 16u + (threadIdx.x & 15u), y = (blk / V.pblk_x) * 16u + (threadIdx.x >> 4);
+  if (x >= fb.pw || y >= fb.ph) return;
+  plain::Ctx C;
+  C.S = S;
+  C.cam = V.cam;
+  C.grass_taper = nullptr;
+  plain::composite_taa_pixel(C, fb, x, y);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -337,5 +468,19 @@ hipError_t vx_launch_composite(const vxd::Scene& S, const vx_camera* cam_dev, co
 }
 hipError_t vx_launch_composite_taa(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, hipStream_t st) {
   hipLaunchKernelGGL(k_composite_taa, dim3((fb.pw + 15u) / 16u, (fb.ph + 15u) / 16u), dim3(256), 0, st, S, cam_dev, fb);
+  return hipGetLastError();
+}
+
+// Fused passes of a batch in reference order; `persistent` is the number of
+// workgroups of the two work-stealing kernels.
+hipError_t vx_launch_batch(const vxd::Scene& S, const vxd::Scene& S_godray, const vxd::Batch& B, uint32_t mask,
+                           uint32_t persistent, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, B);
+  if (mask & VX_PASS_PRIMARY) hipLaunchKernelGGL(k_primary_batch, dim3(persistent), dim3(64), 0, st, S, B);
+  if (mask & VX_PASS_LOCAL_TAA) hipLaunchKernelGGL(k_local_taa_batch, dim3(B.rblks), dim3(256), 0, st, B);
+  if (mask & VX_PASS_GODRAY) hipLaunchKernelGGL(k_godray_batch, dim3(persistent), dim3(64), 0, st, S_godray, B);
+  if (mask & VX_PASS_COMPOSITE) hipLaunchKernelGGL(k_composite_batch, dim3(B.pblks), dim3(256), 0, st, S, B);
+  if (mask & VX_PASS_COMPOSITE_TAA) hipLaunchKernelGGL(k_composite_taa_batch, dim3(B.pblks), dim3(256), 0, st, S, B);
+  if (mask & VX_PASS_END_FRAME) hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
   return hipGetLastError();
 }

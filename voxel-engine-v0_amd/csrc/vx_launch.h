@@ -17,3 +17,7 @@ hipError_t vx_launch_godray(const vxd::Scene& S, const vx_camera* cam_dev, const
                             unsigned long long* counters, hipStream_t st);
 hipError_t vx_launch_composite(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, hipStream_t st);
 hipError_t vx_launch_composite_taa(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, hipStream_t st);
+// A batch of views: fused work-stealing primary and godray kernels, flattened
+// streaming passes, and the end-of-frame ping flip, in reference pass order.
+hipError_t vx_launch_batch(const vxd::Scene& S, const vxd::Scene& S_godray, const vxd::Batch& B, uint32_t mask,
+                           uint32_t persistent, hipStream_t st);

@@ -10,7 +10,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
+#include <numeric>
+#include <vector>
 
 #include "vx_internal.h"
 #include "vx_launch.h"
@@ -74,6 +77,7 @@ struct vx_view {
   int written_primary = 1, written_local = 1, written_godray = 1, written_out = 1;
   hipEvent_t ev[10] = {};
   uint32_t last_mask = 0;
+  uint32_t gen = 0;  // bumped when the view's buffers are reallocated
 };
 
 namespace {
@@ -156,9 +160,11 @@ int alloc_view_buffers(vx_view* v) {
       return rc;
   }
   if ((rc = dalloc(&v->pre_taa, p)) || (rc = dalloc(&v->output, p)) || (rc = dalloc(&v->hits, r))) return rc;
-  if ((rc = dalloc(&v->prim_rec, r)) || (rc = dalloc(&v->prim_count, vxd::kNumPrimQueues))) return rc;
-  for (auto& q : v->prim_q)
-    if ((rc = dalloc(&q, r))) return rc;
+  if (v->ctx->flags & VX_CTX_PRIMARY_STAGED) {  // stage buffers only for the staged primary form (100 B/px)
+    if ((rc = dalloc(&v->prim_rec, r)) || (rc = dalloc(&v->prim_count, vxd::kNumPrimQueues))) return rc;
+    for (auto& q : v->prim_q)
+      if ((rc = dalloc(&q, r))) return rc;
+  }
   v->ping = 0;
   v->written_primary = v->written_local = v->written_godray = v->written_out = 1;
   return VX_OK;
@@ -350,6 +356,7 @@ int vx_view_resize(vx_view* v, uint32_t pw, uint32_t ph, uint32_t rw, uint32_t r
   v->ph = ph;
   v->rw = rw;
   v->rh = rh;
+  ++v->gen;
   return alloc_view_buffers(v);
 }
 
@@ -379,12 +386,8 @@ int vx_write_camera(vx_view* v, const vx_camera* cam) {
   return VX_OK;
 }
 
-int vx_render(vx_view* v, uint32_t mask, void* stream) {
-  if (!v) return vx_fail(VX_E_INVALID, "vx_render: null view");
-  vx_ctx* c = v->ctx;
-  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-  DeviceGuard g(c->device);
-  const int ping = v->ping, pong = 1 - ping;
+static vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits) {
+  const int pong = 1 - ping;
   vxd::FrameBuffers fb;
   std::memset(&fb, 0, sizeof(fb));
   fb.rw = v->rw;
@@ -406,7 +409,17 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
   fb.out_hist_in = v->out_hist[ping];
   fb.out_hist_out = v->out_hist[pong];
   fb.output = v->output;
-  fb.hits = (mask & VX_PASS_WRITE_HITS) ? v->hits : nullptr;
+  fb.hits = hits ? v->hits : nullptr;
+  return fb;
+}
+
+int vx_render(vx_view* v, uint32_t mask, void* stream) {
+  if (!v) return vx_fail(VX_E_INVALID, "vx_render: null view");
+  vx_ctx* c = v->ctx;
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  DeviceGuard g(c->device);
+  const int ping = v->ping, pong = 1 - ping;
+  const vxd::FrameBuffers fb = view_buffers(v, ping, (mask & VX_PASS_WRITE_HITS) != 0);
   const bool count = (mask & VX_PASS_COUNT) != 0;
   const vxd::Scene S = make_scene(c, false);
   VX_HIP(hipEventRecord(v->ev[0], st));
@@ -521,6 +534,235 @@ int vx_view_timings(vx_view* v, float ms[5]) {
     VX_HIP(hipEventElapsedTime(&t, v->ev[i], v->ev[i + 1]));
     ms[i] = t;
   }
+  return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: many views rendered by fused launches (one kernel per pass over
+// all views), optionally replayed from a captured hipGraph.
+// ---------------------------------------------------------------------------
+struct vx_batch {
+  vx_ctx* ctx = nullptr;
+  std::vector<vx_view*> views;      // table order (largest render first)
+  std::vector<uint32_t> gens;       // view buffer generations at creation
+  std::vector<uint32_t> ping_host;  // mirror of the device ping array
+  vxd::BatchView* table = nullptr;
+  uint32_t* ping = nullptr;
+  uint32_t* cursor = nullptr;
+  vxd::Batch B{};
+  vx_batch_config cfg{6u, 1u, 1u, 0u};
+  std::map<uint64_t, hipGraphExec_t> graphs;  // by (mask, config)
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+uint32_t persistent_groups(const vx_batch_config& cfg) {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) cus = n;
+  }
+  return (uint32_t)cus * 4u * std::max(1u, cfg.waves_per_simd);
+}
+void drop_graphs(vx_batch* b) {
+  for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+  b->graphs.clear();
+}
+}  // namespace
+
+int vx_batch_create(vx_ctx* ctx, vx_view* const* views, uint32_t n, vx_batch** out) {
+  if (!ctx || !views || !n || !out) return vx_fail(VX_E_INVALID, "vx_batch_create: bad argument");
+  *out = nullptr;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!views[i] || views[i]->ctx != ctx) return vx_fail(VX_E_INVALID, "vx_batch_create: view %u is null or of another context", i);
+    for (uint32_t j = 0; j < i; ++j)
+      if (views[j] == views[i]) return vx_fail(VX_E_INVALID, "vx_batch_create: view %u appears twice", i);
+  }
+  DeviceGuard g(ctx->device);
+  vx_batch* b = new (std::nothrow) vx_batch();
+  if (!b) return vx_fail(VX_E_NOMEM, "vx_batch_create: host allocation");
+  b->ctx = ctx;
+  b->views.assign(views, views + n);
+  // Largest views first: their tiles are taken first, small views fill the tail.
+  std::stable_sort(b->views.begin(), b->views.end(),
+                   [](const vx_view* a, const vx_view* c) { return (uint64_t)a->rw * a->rh > (uint64_t)c->rw * c->rh; });
+  std::vector<vxd::BatchView> tab(n);
+  uint32_t tiles = 0, rblks = 0, pblks = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_view* v = b->views[i];
+    vxd::BatchView& e = tab[i];
+    std::memset(&e, 0, sizeof(e));
+    e.fb[0] = view_buffers(v, 0, false);
+    e.fb[1] = view_buffers(v, 1, false);
+    e.cam = v->cam_dev;
+    e.tiles_x = (v->rw + 7u) / 8u;
+    e.tiles = e.tiles_x * ((v->rh + 7u) / 8u);
+    e.rblk_x = (v->rw + 15u) / 16u;
+    e.rblks = e.rblk_x * ((v->rh + 15u) / 16u);
+    e.pblk_x = (v->pw + 15u) / 16u;
+    e.pblks = e.pblk_x * ((v->ph + 15u) / 16u);
+    e.tile_base = tiles;
+    e.rblk_base = rblks;
+    e.pblk_base = pblks;
+    tiles += e.tiles;
+    rblks += e.rblks;
+    pblks += e.pblks;
+    b->gens.push_back(v->gen);
+    b->ping_host.push_back((uint32_t)v->ping);
+  }
+  int rc = dalloc(&b->table, n);
+  if (!rc) rc = dalloc(&b->ping, n);
+  if (!rc) rc = dalloc(&b->cursor, 2u * vxd::kBatchBands);
+  for (int i = 0; i < 2 && !rc; ++i)
+    if (hipEventCreate(&b->ev[i]) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "vx_batch_create: hipEventCreate");
+  if (!rc && hipMemcpy(b->table, tab.data(), n * sizeof(vxd::BatchView), hipMemcpyHostToDevice) != hipSuccess)
+    rc = vx_fail(VX_E_DEVICE, "vx_batch_create: table upload");
+  if (!rc && hipMemcpy(b->ping, b->ping_host.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+    rc = vx_fail(VX_E_DEVICE, "vx_batch_create: ping upload");
+  if (rc) {
+    vx_batch_destroy(b);
+    return rc;
+  }
+  b->B.views = b->table;
+  b->B.ping = b->ping;
+  b->B.cursor = b->cursor;
+  b->B.n = n;
+  b->B.tiles = tiles;
+  b->B.rblks = rblks;
+  b->B.pblks = pblks;
+  b->B.grab = b->cfg.grab;
+  *out = b;
+  return VX_OK;
+}
+
+void vx_batch_destroy(vx_batch* b) {
+  if (!b) return;
+  DeviceGuard g(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  drop_graphs(b);
+  dfree(b->table);
+  dfree(b->ping);
+  dfree(b->cursor);
+  for (auto& e : b->ev)
+    if (e) hipEventDestroy(e);
+  delete b;
+}
+
+int vx_batch_set_config(vx_batch* b, const vx_batch_config* cfg) {
+  if (!b || !cfg) return vx_fail(VX_E_INVALID, "vx_batch_set_config: null argument");
+  if (cfg->waves_per_simd < 1u || cfg->waves_per_simd > 16u || cfg->grab < 1u || cfg->grab > 64u)
+    return vx_fail(VX_E_INVALID, "vx_batch_set_config: waves_per_simd in 1..16, grab in 1..64");
+  b->cfg = *cfg;
+  b->B.grab = cfg->grab;
+  return VX_OK;
+}
+
+int vx_batch_get_config(const vx_batch* b, vx_batch_config* cfg) {
+  if (!b || !cfg) return vx_fail(VX_E_INVALID, "vx_batch_get_config: null argument");
+  *cfg = b->cfg;
+  return VX_OK;
+}
+
+int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
+  if (!b) return vx_fail(VX_E_INVALID, "vx_batch_render: null batch");
+  if (mask & (VX_PASS_WRITE_HITS | VX_PASS_COUNT)) return vx_fail(VX_E_INVALID, "vx_batch_render: hits and counting run per view");
+  vx_ctx* c = b->ctx;
+  DeviceGuard g(c->device);
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  const uint32_t n = (uint32_t)b->views.size();
+  bool ping_changed = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (b->views[i]->gen != b->gens[i]) return vx_fail(VX_E_INVALID, "vx_batch_render: a view was resized; recreate the batch");
+    if (b->ping_host[i] != (uint32_t)b->views[i]->ping) {
+      b->ping_host[i] = (uint32_t)b->views[i]->ping;
+      ping_changed = true;
+    }
+  }
+  if (ping_changed) {  // a view was rendered on its own since the last batch
+    VX_HIP(hipMemcpyAsync(b->ping, b->ping_host.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    VX_HIP(hipStreamSynchronize(st));
+  }
+  const vxd::Scene S = make_scene(c, false), SG = make_scene(c, true);
+  const uint32_t persistent = persistent_groups(b->cfg);
+  VX_HIP(hipEventRecord(b->ev[0], st));
+  if (b->cfg.use_graph) {
+    const uint64_t key = ((uint64_t)mask << 32) | ((uint64_t)b->cfg.waves_per_simd << 16) | b->cfg.grab;
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      hipStream_t cap = nullptr;
+      VX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        hipError_t le = vx_launch_batch(S, SG, b->B, mask, persistent, cap);
+        e = hipStreamEndCapture(cap, &graph);
+        if (e == hipSuccess) e = le;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (graph) hipGraphDestroy(graph);
+      hipStreamDestroy(cap);
+      if (e != hipSuccess) return vx_fail(VX_E_DEVICE, "vx_batch_render: graph capture: %s", hipGetErrorString(e));
+      it = b->graphs.emplace(key, exec).first;
+    }
+    VX_HIP(hipGraphLaunch(it->second, st));
+  } else {
+    VX_HIP(vx_launch_batch(S, SG, b->B, mask, persistent, st));
+  }
+  VX_HIP(hipEventRecord(b->ev[1], st));
+  for (uint32_t i = 0; i < n; ++i) {
+    vx_view* v = b->views[i];
+    const int pong = 1 - v->ping;
+    if (mask & VX_PASS_PRIMARY) v->written_primary = pong;
+    if (mask & VX_PASS_LOCAL_TAA) v->written_local = pong;
+    if (mask & VX_PASS_GODRAY) v->written_godray = pong;
+    if (mask & VX_PASS_COMPOSITE_TAA) v->written_out = pong;
+    v->last_mask = mask;
+    if (mask & VX_PASS_END_FRAME) {
+      v->ping = pong;
+      b->ping_host[i] = (uint32_t)pong;
+    }
+  }
+  return VX_OK;
+}
+
+int vx_batch_timing(vx_batch* b, float* ms) {
+  if (!b || !ms) return vx_fail(VX_E_INVALID, "vx_batch_timing: null argument");
+  DeviceGuard g(b->ctx->device);
+  VX_HIP(hipEventSynchronize(b->ev[1]));
+  VX_HIP(hipEventElapsedTime(ms, b->ev[0], b->ev[1]));
+  return VX_OK;
+}
+
+int vx_batch_tune(vx_batch* b, uint32_t mask, uint32_t iters, vx_batch_config* chosen) {
+  if (!b) return vx_fail(VX_E_INVALID, "vx_batch_tune: null batch");
+  mask &= ~(uint32_t)VX_PASS_END_FRAME;  // re-running the passes without the flip leaves the history as it was
+  if (!(mask & (VX_PASS_PRIMARY | VX_PASS_GODRAY))) return vx_fail(VX_E_INVALID, "vx_batch_tune: no work-stealing pass in mask");
+  iters = std::max(1u, iters);
+  const vx_batch_config keep = b->cfg;
+  vx_batch_config best = keep;
+  float best_ms = 1e30f;
+  const uint32_t waves[] = {4u, 6u, 8u};
+  const uint32_t grabs[] = {1u, 2u};
+  for (uint32_t w : waves)
+    for (uint32_t gsz : grabs) {
+      vx_batch_config cfg{w, gsz, keep.use_graph, 0u};
+      int rc = vx_batch_set_config(b, &cfg);
+      if (rc) return rc;
+      float total = 0.0f;
+      for (uint32_t it = 0; it <= iters; ++it) {
+        if ((rc = vx_batch_render(b, mask, nullptr))) return rc;
+        float ms = 0.0f;
+        if ((rc = vx_batch_timing(b, &ms))) return rc;
+        if (it > 0) total += ms;  // the first run warms up (and captures the graph)
+      }
+      if (total < best_ms) {
+        best_ms = total;
+        best = cfg;
+      }
+    }
+  vx_batch_set_config(b, &best);
+  if (chosen) *chosen = best;
   return VX_OK;
 }
 

@@ -65,6 +65,26 @@ struct PrimQueues {
   uint32_t* __restrict__ count;  // kNumPrimQueues counters
 };
 
+// A fused multi-view launch: one table entry per view (both history sets, so
+// the table stays valid while the views' ping-pong flips frame to frame),
+// prefix offsets of the views' work items, and the work-stealing cursors.
+struct BatchView {
+  FrameBuffers fb[2];                       // by the view's current ping
+  const vx_camera* cam;
+  uint32_t tiles_x, tiles, tile_base;       // 8x8 tiles at render resolution
+  uint32_t rblk_x, rblks, rblk_base;        // 16x16 blocks at render resolution
+  uint32_t pblk_x, pblks, pblk_base;        // 16x16 blocks at present resolution
+  uint32_t pad;
+};
+constexpr uint32_t kBatchBands = 8;         // one tile band per XCD
+struct Batch {
+  const BatchView* __restrict__ views;
+  uint32_t* __restrict__ ping;              // per view, flipped by the end-of-frame kernel
+  uint32_t* __restrict__ cursor;            // [pass][band]: next tile of each band (primary, godray)
+  uint32_t n, tiles, rblks, pblks;
+  uint32_t grab;                            // tiles taken per atomic
+};
+
 // Workgroup-shared primary candidate list (common.wgsl:422-424).
 constexpr uint32_t kMaxTileChunks = 192;
 struct TileLDS {

@@ -107,6 +107,13 @@ class Pose(C.Structure):
                 ("z_near", C.c_float), ("z_far", C.c_float)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("waves_per_simd", C.c_uint32), ("grab", C.c_uint32), ("use_graph", C.c_uint32), ("pad", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {"waves_per_simd": self.waves_per_simd, "grab": self.grab, "use_graph": self.use_graph}
+
+
 class HostWorldView(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("ropes", C.c_void_p), ("meta", C.c_void_p),
                 ("chunk_count", C.c_uint32), ("macro_words", C.c_void_p), ("colinfo_words", C.c_void_p),
@@ -142,6 +149,11 @@ def _load() -> C.CDLL:
         "vx_view_timings": ([vp, C.POINTER(f32)], C.c_int),
         "vx_query_rays": ([vp, C.POINTER(Camera), vp, u32, vp, vp], C.c_int),
         "vx_sync": ([vp], C.c_int),
+        "vx_batch_create": ([vp, C.POINTER(vp), u32, C.POINTER(vp)], C.c_int), "vx_batch_destroy": ([vp], None),
+        "vx_batch_render": ([vp, u32, vp], C.c_int), "vx_batch_timing": ([vp, C.POINTER(f32)], C.c_int),
+        "vx_batch_set_config": ([vp, C.POINTER(BatchConfig)], C.c_int),
+        "vx_batch_get_config": ([vp, C.POINTER(BatchConfig)], C.c_int),
+        "vx_batch_tune": ([vp, u32, u32, C.POINTER(BatchConfig)], C.c_int),
         "vx_build_chunk": ([vp, vp, i32, i32, vp, vp, u32, C.POINTER(u32), vp, vp], C.c_int),
         "vx_world_default": ([C.POINTER(WorldParams)], None),
         "vx_world_ground_height": ([C.POINTER(WorldParams), i32, i32], i32),
@@ -166,7 +178,8 @@ EXPORTED = (
     "vx_last_error", "vx_version", "vx_device_count", "vx_create", "vx_destroy", "vx_upload_regions", "vx_write_grid",
     "vx_write_clipmap", "vx_view_create", "vx_view_destroy", "vx_view_resize", "vx_view_reset", "vx_write_camera",
     "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings",
-    "vx_query_rays", "vx_sync", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
+    "vx_query_rays", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
+    "vx_batch_set_config", "vx_batch_get_config", "vx_batch_tune", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
     "vx_host_world_build", "vx_host_world_free", "vx_host_world_get", "vx_host_world_edit", "vx_camera_build",
     "vx_clipmap_build",
 )
@@ -427,6 +440,51 @@ class View:
     def close(self) -> None:
         if self._h:
             LIB.vx_view_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Views of one context rendered together by fused launches (vx_batch_*)."""
+
+    def __init__(self, ctx: Context, views):
+        self.ctx = ctx
+        self.views = list(views)  # keeps the views alive while the batch refers to them
+        arr = (C.c_void_p * len(self.views))(*[v._h.value for v in self.views])
+        h = C.c_void_p()
+        _check(LIB.vx_batch_create(ctx._h, arr, len(self.views), C.byref(h)), "vx_batch_create")
+        self._h = h
+
+    def render(self, mask: int = PASS_FRAME, stream: int | None = None) -> None:
+        _check(LIB.vx_batch_render(self._h, mask, stream), "vx_batch_render")
+
+    def timing(self) -> float:
+        ms = C.c_float()
+        _check(LIB.vx_batch_timing(self._h, C.byref(ms)), "vx_batch_timing")
+        return float(ms.value)
+
+    def config(self) -> BatchConfig:
+        cfg = BatchConfig()
+        _check(LIB.vx_batch_get_config(self._h, C.byref(cfg)), "vx_batch_get_config")
+        return cfg
+
+    def set_config(self, waves_per_simd: int = 6, grab: int = 1, use_graph: bool = True) -> None:
+        cfg = BatchConfig(waves_per_simd, grab, 1 if use_graph else 0, 0)
+        _check(LIB.vx_batch_set_config(self._h, C.byref(cfg)), "vx_batch_set_config")
+
+    def tune(self, mask: int = PASS_FRAME, iters: int = 2) -> BatchConfig:
+        cfg = BatchConfig()
+        _check(LIB.vx_batch_tune(self._h, mask, iters, C.byref(cfg)), "vx_batch_tune")
+        return cfg
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

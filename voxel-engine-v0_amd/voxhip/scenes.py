@@ -105,9 +105,13 @@ def c1(seed: int = 42, frames: int = 120, resolution=(1920, 1080)) -> Scene:
 
 
 def frame_camera(scene: Scene, world: HostWorld, frame: int, prev_view_proj=None, *, wind_strength: float = 2.0,
-                 time_s: float | None = None, jitter: bool = False) -> Camera:
+                 time_s: float | None = None, jitter: bool = False, render=None, present=None) -> Camera:
+    """Camera of `frame` of the scene's flight, at the scene's resolution unless
+    render (and present, default = render) are given."""
     pose, t = scene.poses[frame % len(scene.poses)]
-    return camera_build(pose, scene.render, scene.present, frame, time_s=t if time_s is None else time_s,
+    render = render or scene.render
+    present = present or (render if render is not scene.render else scene.present)
+    return camera_build(pose, render, present, frame, time_s=t if time_s is None else time_s,
                         wind_strength=wind_strength, fog_density=0.002, grid_origin=scene.grid_origin,
                         grid_dims=scene.grid_dims, chunk_count=world.chunk_count, prev_view_proj=prev_view_proj,
                         jitter=jitter)
