@@ -370,22 +370,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
   }
 }
 
+// Godray: one workgroup per tile of every view (a flattened grid; the pass is
+// short and even, so plain dispatch beats work stealing), XCD bands as in k_godray.
 __global__ __launch_bounds__(64) void k_godray_batch(Scene S, Batch B) {
-  uint32_t band = blockIdx.x % kBatchBands, steps = 0u, t0 = 0u, t1 = 0u;
-  while (batch_take(B, 1u, band, steps, t0, t1)) {
-    for (uint32_t t = t0; t < t1; ++t) {
-      const uint32_t v = batch_view_of(B, t, 0);
-      const BatchView& V = B.views[v];
-      const FrameBuffers& fb = V.fb[B.ping[v] & 1u];
-      const uint32_t tile = t - V.tile_base;
-      const uint32_t gx = (tile % V.tiles_x) * 8u + (threadIdx.x & 7u), gy = (tile / V.tiles_x) * 8u + (threadIdx.x >> 3);
-      plain::Ctx C;
-      C.S = S;
-      C.cam = V.cam;
-      C.grass_taper = nullptr;
-      if (gx < fb.rw && gy < fb.rh) plain::godray_pixel(C, fb, gx, gy);
-    }
-  }
+  const uint32_t t = xcd_tile(blockIdx.x, B.tiles);
+  const uint32_t v = batch_view_of(B, t, 0);
+  const BatchView& V = B.views[v];
+  const FrameBuffers& fb = V.fb[B.ping[v] & 1u];
+  const uint32_t tile = t - V.tile_base;
+  const uint32_t gx = (tile % V.tiles_x) * 8u + (threadIdx.x & 7u), gy = (tile / V.tiles_x) * 8u + (threadIdx.x >> 3);
+  plain::Ctx C;
+  C.S = S;
+  C.cam = V.cam;
+  C.grass_taper = nullptr;
+  if (gx < fb.rw && gy < fb.rh) plain::godray_pixel(C, fb, gx, gy);
 }
 
 __global__ __launch_bounds__(256) void k_local_taa_batch(Batch B) {
@@ -478,9 +476,19 @@ hipError_t vx_launch_batch(const vxd::Scene& S, const vxd::Scene& S_godray, cons
   hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, B);
   if (mask & VX_PASS_PRIMARY) hipLaunchKernelGGL(k_primary_batch, dim3(persistent), dim3(64), 0, st, S, B);
   if (mask & VX_PASS_LOCAL_TAA) hipLaunchKernelGGL(k_local_taa_batch, dim3(B.rblks), dim3(256), 0, st, B);
-  if (mask & VX_PASS_GODRAY) hipLaunchKernelGGL(k_godray_batch, dim3(persistent), dim3(64), 0, st, S_godray, B);
+  if (mask & VX_PASS_GODRAY) hipLaunchKernelGGL(k_godray_batch, dim3(B.tiles), dim3(64), 0, st, S_godray, B);
   if (mask & VX_PASS_COMPOSITE) hipLaunchKernelGGL(k_composite_batch, dim3(B.pblks), dim3(256), 0, st, S, B);
   if (mask & VX_PASS_COMPOSITE_TAA) hipLaunchKernelGGL(k_composite_taa_batch, dim3(B.pblks), dim3(256), 0, st, S, B);
   if (mask & VX_PASS_END_FRAME) hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
+  return hipGetLastError();
+}
+
+// One view through the work-stealing kernels (vx_render's default form).
+hipError_t vx_launch_ws_begin(const vxd::Batch& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, B);
+  return hipGetLastError();
+}
+hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, hipStream_t st) {
+  hipLaunchKernelGGL(k_primary_batch, dim3(persistent), dim3(64), 0, st, S, B);
   return hipGetLastError();
 }

@@ -21,3 +21,7 @@ hipError_t vx_launch_composite_taa(const vxd::Scene& S, const vx_camera* cam_dev
 // streaming passes, and the end-of-frame ping flip, in reference pass order.
 hipError_t vx_launch_batch(const vxd::Scene& S, const vxd::Scene& S_godray, const vxd::Batch& B, uint32_t mask,
                            uint32_t persistent, hipStream_t st);
+// One view through the same work-stealing primary kernel: reset the tile
+// cursors, then a persistent launch of `persistent` workgroups.
+hipError_t vx_launch_ws_begin(const vxd::Batch& B, hipStream_t st);
+hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, hipStream_t st);

@@ -69,6 +69,10 @@ struct vx_view {
   float4* out_hist[2] = {nullptr, nullptr};
   vx_hit* hits = nullptr;
   // Primary-pass stage buffers: voxel-hit records, pixel queues, counters.
+  // One-view work-stealing launch: table [no hits, hits] and a small buffer
+  // {0, 1, cursors[2 * 8]} whose first two words serve as the ping value.
+  vxd::BatchView* ws_tab = nullptr;
+  uint32_t* ws_buf = nullptr;
   vxd::PrimRec* prim_rec = nullptr;
   uint32_t* prim_q[vxd::kNumPrimQueues] = {};
   uint32_t* prim_count = nullptr;
@@ -148,7 +152,12 @@ void free_view_buffers(vx_view* v) {
   dfree(v->prim_rec);
   for (auto& q : v->prim_q) dfree(q);
   dfree(v->prim_count);
+  dfree(v->ws_tab);
+  dfree(v->ws_buf);
 }
+
+vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits);
+vxd::BatchView one_view_entry(const vx_view* v, bool hits);
 
 int alloc_view_buffers(vx_view* v) {
   const size_t r = (size_t)v->rw * v->rh, p = (size_t)v->pw * v->ph;
@@ -165,6 +174,13 @@ int alloc_view_buffers(vx_view* v) {
     for (auto& q : v->prim_q)
       if ((rc = dalloc(&q, r))) return rc;
   }
+  if (!v->cam_dev) return vx_fail(VX_E_INVALID, "view tables need the camera buffer");
+  if ((rc = dalloc(&v->ws_tab, 2)) || (rc = dalloc(&v->ws_buf, 2 + 2 * vxd::kBatchBands))) return rc;
+  const vxd::BatchView tab[2] = {one_view_entry(v, false), one_view_entry(v, true)};
+  const uint32_t pings[2] = {0u, 1u};
+  if (hipMemcpy(v->ws_tab, tab, sizeof(tab), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(v->ws_buf, pings, sizeof(pings), hipMemcpyHostToDevice) != hipSuccess)
+    return vx_fail(VX_E_DEVICE, "view table upload");
   v->ping = 0;
   v->written_primary = v->written_local = v->written_godray = v->written_out = 1;
   return VX_OK;
@@ -322,8 +338,9 @@ int vx_view_create(vx_ctx* ctx, uint32_t pw, uint32_t ph, uint32_t rw, uint32_t 
   v->rw = rw;
   v->rh = rh;
   DeviceGuard g(ctx->device);
-  int rc = alloc_view_buffers(v);
-  if (!rc) rc = dalloc(&v->cam_dev, 1);
+  // The camera buffer first: the view's kernel table (built with the buffers) points at it.
+  int rc = dalloc(&v->cam_dev, 1);
+  if (!rc) rc = alloc_view_buffers(v);
   if (!rc) rc = dalloc(&v->counters, 16);
   for (int i = 0; i < 10 && !rc; ++i)
     if (hipEventCreate(&v->ev[i]) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "vx_view_create: hipEventCreate");
@@ -386,7 +403,40 @@ int vx_write_camera(vx_view* v, const vx_camera* cam) {
   return VX_OK;
 }
 
-static vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits) {
+}  // extern "C"
+
+struct vx_batch {
+  vx_ctx* ctx = nullptr;
+  std::vector<vx_view*> views;      // table order (largest render first)
+  std::vector<uint32_t> gens;       // view buffer generations at creation
+  std::vector<uint32_t> ping_host;  // mirror of the device ping array
+  vxd::BatchView* table = nullptr;
+  uint32_t* ping = nullptr;
+  uint32_t* cursor = nullptr;
+  vxd::Batch B{};
+  vx_batch_config cfg{6u, 1u, 1u, 0u};
+  std::map<uint64_t, hipGraphExec_t> graphs;  // by (mask, config)
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+uint32_t persistent_groups(const vx_batch_config& cfg) {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) cus = n;
+  }
+  return (uint32_t)cus * 4u * std::max(1u, cfg.waves_per_simd);
+}
+void drop_graphs(vx_batch* b) {
+  for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+  b->graphs.clear();
+}
+}  // namespace
+
+namespace {
+
+vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits) {
   const int pong = 1 - ping;
   vxd::FrameBuffers fb;
   std::memset(&fb, 0, sizeof(fb));
@@ -413,6 +463,25 @@ static vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits) {
   return fb;
 }
 
+vxd::BatchView one_view_entry(const vx_view* v, bool hits) {
+  vxd::BatchView e;
+  std::memset(&e, 0, sizeof(e));
+  e.fb[0] = view_buffers(v, 0, hits);
+  e.fb[1] = view_buffers(v, 1, hits);
+  e.cam = v->cam_dev;
+  e.tiles_x = (v->rw + 7u) / 8u;
+  e.tiles = e.tiles_x * ((v->rh + 7u) / 8u);
+  e.rblk_x = (v->rw + 15u) / 16u;
+  e.rblks = e.rblk_x * ((v->rh + 15u) / 16u);
+  e.pblk_x = (v->pw + 15u) / 16u;
+  e.pblks = e.pblk_x * ((v->ph + 15u) / 16u);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
 int vx_render(vx_view* v, uint32_t mask, void* stream) {
   if (!v) return vx_fail(VX_E_INVALID, "vx_render: null view");
   vx_ctx* c = v->ctx;
@@ -422,10 +491,24 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
   const vxd::FrameBuffers fb = view_buffers(v, ping, (mask & VX_PASS_WRITE_HITS) != 0);
   const bool count = (mask & VX_PASS_COUNT) != 0;
   const vxd::Scene S = make_scene(c, false);
+  // The primary pass runs as a persistent work-stealing kernel over the view's
+  // tiles (the one-view case of a batch); the counting form is one workgroup
+  // per tile.
+  vxd::Batch B1{};
+  B1.views = v->ws_tab + ((mask & VX_PASS_WRITE_HITS) ? 1 : 0);
+  B1.ping = v->ws_buf + ping;  // ws_buf[0] = 0, ws_buf[1] = 1
+  B1.cursor = v->ws_buf + 2;
+  B1.n = 1;
+  B1.tiles = ((v->rw + 7u) / 8u) * ((v->rh + 7u) / 8u);
+  B1.grab = 1;
+  const uint32_t persistent = persistent_groups(vx_batch_config{8u, 1u, 0u, 0u});
   VX_HIP(hipEventRecord(v->ev[0], st));
+  if (!count && (mask & VX_PASS_PRIMARY) && !(c->flags & VX_CTX_PRIMARY_STAGED)) VX_HIP(vx_launch_ws_begin(B1, st));
   if (mask & VX_PASS_PRIMARY) {
-    if (count || !(c->flags & VX_CTX_PRIMARY_STAGED)) {
-      VX_HIP(vx_launch_primary(S, v->cam_dev, fb, count, v->counters, st));
+    if (count) {
+      VX_HIP(vx_launch_primary(S, v->cam_dev, fb, true, v->counters, st));
+    } else if (!(c->flags & VX_CTX_PRIMARY_STAGED)) {
+      VX_HIP(vx_launch_primary_ws(S, B1, persistent, st));
     } else {
       vxd::PrimQueues Q;
       Q.rec = v->prim_rec;
@@ -541,34 +624,7 @@ int vx_view_timings(vx_view* v, float ms[5]) {
 // Batches: many views rendered by fused launches (one kernel per pass over
 // all views), optionally replayed from a captured hipGraph.
 // ---------------------------------------------------------------------------
-struct vx_batch {
-  vx_ctx* ctx = nullptr;
-  std::vector<vx_view*> views;      // table order (largest render first)
-  std::vector<uint32_t> gens;       // view buffer generations at creation
-  std::vector<uint32_t> ping_host;  // mirror of the device ping array
-  vxd::BatchView* table = nullptr;
-  uint32_t* ping = nullptr;
-  uint32_t* cursor = nullptr;
-  vxd::Batch B{};
-  vx_batch_config cfg{6u, 1u, 1u, 0u};
-  std::map<uint64_t, hipGraphExec_t> graphs;  // by (mask, config)
-  hipEvent_t ev[2] = {nullptr, nullptr};
-};
 
-namespace {
-uint32_t persistent_groups(const vx_batch_config& cfg) {
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess) {
-    int n = 0;
-    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) cus = n;
-  }
-  return (uint32_t)cus * 4u * std::max(1u, cfg.waves_per_simd);
-}
-void drop_graphs(vx_batch* b) {
-  for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
-  b->graphs.clear();
-}
-}  // namespace
 
 int vx_batch_create(vx_ctx* ctx, vx_view* const* views, uint32_t n, vx_batch** out) {
   if (!ctx || !views || !n || !out) return vx_fail(VX_E_INVALID, "vx_batch_create: bad argument");
