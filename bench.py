@@ -43,6 +43,8 @@ def parse():
                     help="views of the mixed-resolution batching workload (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-1080p", action="store_true", help="skip the 1080p per-view latency run")
+    ap.add_argument("--query-batches", type=int, default=256,
+                    help="ray batches of the incoherent ray-query workload (0: skip it)")
     ap.add_argument("--single-batch", action="store_true",
                     help="render the 4K view through a one-view batch (work-stealing launches) instead of vx_render")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline)")
@@ -173,6 +175,7 @@ def main():
 
     lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup)
     batched = None if a.batch_views <= 0 else batching(vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps)
+    queries = None if a.query_batches <= 0 else ray_queries(vx, scenes, a.query_batches)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -203,6 +206,7 @@ def main():
         "cpu_baseline": cpu,
         "latency_1080p_ms": lat1080,
         "batched": batched,
+        "ray_queries": queries,
         "setup_s": round(setup_s, 2),
     }
     if counts is not None:
@@ -288,6 +292,54 @@ def batching(vx, scenes, sc, hw, ctx, n_views, steps):
             "plain_rays_per_s": round(rays / t_plain, 1), "batched_rays_per_s": round(rays / t_batch, 1),
             "speedup": round(t_plain / t_batch, 3), "plain_ms_per_step": round(t_plain * 1e3, 2),
             "batched_ms_per_step": round(t_batch * 1e3, 2), "tuned": tuned}
+
+
+def ray_queries(vx, scenes, n_batches):
+    """The incoherent ray-query workload: n_batches batches of log-uniform
+    sizes in [1e3, 1e6] rays (seed 11), origins uniform in the 32x4x32-chunk
+    world's AABB, directions uniform on the sphere, max distance 512 voxels.
+    Fused = all batches in one vx_query_rays launch (after the autotuner);
+    plain = one launch per batch."""
+    rng = np.random.default_rng(11)
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n_batches)).astype(np.int64)
+    n = int(sizes.sum())
+    dims, origin = (32, 4, 32), (0, 0, 0)
+    t0 = time.time()
+    hw = vx.HostWorld(vx.world_params(42), origin, dims)
+    build_s = time.time() - t0
+    ctx = vx.Context(node_capacity=len(hw.nodes), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
+                     clip_levels=1, clip_res=256)
+    ctx.upload_world(hw)
+    cam = vx.camera_build(vx.make_pose((1.0, 1.0, 1.0), 0.0, 0.0), (64, 64), (64, 64), 0, grid_origin=origin,
+                          grid_dims=dims, chunk_count=hw.chunk_count)
+    rays = scenes.query_rays(rng, n, origin, dims, tmax_m=51.2)
+    d_rays = vx.DeviceArray.from_numpy(rays)
+    d_hits = vx.DeviceArray(n * vx.HIT_DTYPE.itemsize)
+    tuned = vx.query_tune(ctx, cam, d_rays, d_hits, n).as_dict()
+    vx.query_rays_device(ctx, cam, d_rays, d_hits, n)  # warm
+    ctx.sync()
+    t = time.perf_counter()
+    vx.query_rays_device(ctx, cam, d_rays, d_hits, n)
+    ctx.sync()
+    t_fused = time.perf_counter() - t
+    gpu_fused_ms = vx.query_timing(ctx)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    stride = vx.HIT_DTYPE.itemsize
+    t = time.perf_counter()
+    for i in range(n_batches):
+        k = int(sizes[i])
+        vx.query_rays_device(ctx, cam, d_rays.view(int(offs[i]) * 32, k * 32), d_hits.view(int(offs[i]) * stride, k * stride), k)
+    ctx.sync()
+    t_plain = time.perf_counter() - t
+    hits = d_hits.download(np.empty(n, vx.HIT_DTYPE))
+    d_rays.close()
+    d_hits.close()
+    return {"workload": f"{n_batches} batches, log-uniform 1e3..1e6 rays (seed 11), 32x4x32-chunk world, origins uniform "
+                        f"in its AABB, directions uniform, tmax 51.2 m", "rays": n, "chunks": hw.chunk_count,
+            "world_build_s": round(build_s, 2), "fused_rays_per_s": round(n / t_fused, 1),
+            "fused_gpu_ms": round(gpu_fused_ms, 3), "plain_rays_per_s": round(n / t_plain, 1),
+            "speedup": round(t_plain / t_fused, 3), "hit_fraction": round(float(np.mean(hits["kind"] == 1)), 4),
+            "tuned": tuned}
 
 
 def cpu_baseline(sc, hw, clip, heights, cam, W, H, rows):

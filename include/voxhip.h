@@ -259,14 +259,33 @@ int vx_view_counters(vx_view* view, uint64_t out[9], int reset);
 int vx_view_timings(vx_view* view, float ms[5]);
 
 /* Ray queries (no reference counterpart beyond app/app.rs:847-961's CPU pick):
- * first voxel hit of each ray against the resident world, hit identity only.
- * Rays and hits are DEVICE pointers; the call is stream-ordered. */
+ * the first voxel hit of each ray against the resident world within [0, tmax],
+ * voxel geometry only (leaf cubes; no grass blades, no leaf displacement).
+ * scene_cam (host) supplies the grid origin/dims, chunk count and voxel size.
+ * Rays and hits are DEVICE pointers; the call is stream-ordered and returns
+ * without waiting.  All n rays run as ONE persistent work-stealing launch in
+ * which lanes whose ray finished take new rays (lane refill).  Hit: kind 1,
+ * slot, leaf node (chunk-local), voxel (world voxel coords of the leaf's
+ * minimum corner), mat, t.  Miss: kind 0, slot/node 0xFFFFFFFF. */
 typedef struct vx_ray {
     float ox, oy, oz, tmax;   /* origin (metres), max distance (metres) */
     float dx, dy, dz, pad;    /* direction (unit) */
 } vx_ray;
 int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev,
                   uint32_t n, vx_hit* hits_dev, void* stream);
+/* GPU milliseconds of the last vx_query_rays (waits for it). */
+int vx_query_timing(vx_ctx* ctx, float* ms);
+typedef struct vx_query_config {
+    uint32_t waves_per_simd;  /* persistent workgroups = CUs * 4 SIMDs * this (default 8) */
+    uint32_t refill_below;    /* refill idle lanes when fewer lanes than this are busy (default 48) */
+    uint32_t pad[2];
+} vx_query_config;
+int vx_query_set_config(vx_ctx* ctx, const vx_query_config* cfg);
+int vx_query_get_config(const vx_ctx* ctx, vx_query_config* cfg);
+/* Autotuner: runs the query for waves_per_simd in {4, 6, 8} x refill_below in
+ * {16, 32, 48, 64}, keeps the fastest (returned in *chosen, may be NULL). */
+int vx_query_tune(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n,
+                  vx_hit* hits_dev, vx_query_config* chosen);
 
 /* Batches: many views of one context rendered together (the serving and
  * batching path; no reference counterpart: the reference renders one view per

@@ -53,6 +53,7 @@ def _load() -> C.CDLL:
     lib.vxo_primary.argtypes = [C.POINTER(Scene), vp, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp, u32, u32, C.c_int,
                                 C.POINTER(Counts)]
     lib.vxo_local_taa.argtypes = [u32, u32, vp, vp, vp]
+    lib.vxo_query_rays.argtypes = [C.POINTER(Scene), vp, vp, u32, vp, C.c_int, C.POINTER(Counts)]
     lib.vxo_godray.argtypes = [C.POINTER(Scene), vp, u32, u32, C.c_int, vp, vp, vp, C.c_int, C.POINTER(Counts)]
     lib.vxo_composite.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp, C.c_int]
     lib.vxo_composite_taa.argtypes = [vp, u32, u32, u32, u32, vp, vp, vp, vp, vp]
@@ -129,6 +130,15 @@ def primary(world: World, cam, fr: Frame, hits_dtype=None, tiles=(0, 0), threads
                          C.byref(counts) if counts is not None else None)
     assert rc == 0
     fr.primary_hist, fr.shadow, fr.hits = hist_out, shadow_out, hits
+
+
+def query_rays(world: World, cam, rays: np.ndarray, hits_dtype, threads: int = 0, counts: Counts | None = None):
+    """First voxel-geometry hit of each ray; rays is an (n, 8) float32 array (ox, oy, oz, tmax, dx, dy, dz, 0)."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    hits = np.zeros(len(rays), hits_dtype)
+    assert LIB.vxo_query_rays(C.byref(world.scene), C.addressof(cam), rays.ctypes.data, len(rays), hits.ctypes.data,
+                              threads, C.byref(counts) if counts is not None else None) == 0
+    return hits
 
 
 def local_taa(fr: Frame):

@@ -102,6 +102,8 @@ struct Ctx {
   // robust-access implementation would; tests assert none happen).
   uint64_t n_chunks, n_nodes, n_grid, n_macro, n_colinfo, n_clip;
   mutable uint64_t oob;
+  // Ray queries trace voxel geometry only: no grass blades, no leaf displacement.
+  bool geom_only = false;
 };
 
 static inline vx_chunk_meta ld_chunk(const Ctx& C, uint32_t slot) {

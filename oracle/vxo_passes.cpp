@@ -1181,6 +1181,35 @@ int vxo_local_taa(uint32_t w, uint32_t h, const float* local_in, const float* hi
   return 0;
 }
 
+// Ray queries: first voxel-geometry hit of each ray (rays as vx_ray, hits as
+// vx_hit: kind 1 = hit, 0 = miss).
+int vxo_query_rays(const vxo_scene* scene, const vx_camera* cam, const vx_ray* rays, uint32_t n, vx_hit* hits, int threads,
+                   vxo_counts* counts) {
+  if (!scene || !cam || (!rays && n) || (!hits && n)) return -1;
+  threads = hw_threads(threads);
+  std::vector<vxo::Counts> cnt((size_t)threads);
+  std::vector<uint64_t> oob((size_t)threads, 0);
+  std::memset(cnt.data(), 0, sizeof(vxo::Counts) * cnt.size());
+  parallel_for(n, threads, [&](uint32_t i, int t) {
+    vxo::Ctx C = make_ctx(scene, cam, &cnt[t], false);
+    C.geom_only = true;
+    const vx_ray& r = rays[i];
+    vxo::HitGeom h = vxo::query_ray(C, vxo::V3(r.ox, r.oy, r.oz), vxo::V3(r.dx, r.dy, r.dz), r.tmax);
+    vx_hit& o = hits[i];
+    o.kind = h.hit ? 1u : 0u;
+    o.slot = h.hit ? h.slot : VX_INVALID_U32;
+    o.node = h.hit ? h.node : VX_INVALID_U32;
+    o.voxel[0] = h.hit ? h.vox[0] : 0;
+    o.voxel[1] = h.hit ? h.vox[1] : 0;
+    o.voxel[2] = h.hit ? h.vox[2] : 0;
+    o.mat = h.hit ? h.mat : VX_MAT_AIR;
+    o.t = h.hit ? h.t : vxo::BIG_F32;
+    oob[t] += C.oob;
+  });
+  for (int t = 0; t < threads; ++t) add_counts(counts, cnt[t], oob[t]);
+  return 0;
+}
+
 int vxo_godray(const vxo_scene* scene, const vx_camera* cam, uint32_t w, uint32_t h, int macro_compat, const float* depth,
                const float* hist, float* out, int threads, vxo_counts* counts) {
   if (!scene || !cam || !w || !h || !depth || !hist || !out) return -1;

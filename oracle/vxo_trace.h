@@ -411,7 +411,7 @@ static inline ChunkTraceResult trace_chunk_rope_interval_nomacro(const Ctx& C, v
       float limit = grass_probe_max_leaf;
       if (lod_probe == 1u) limit = 4.0f * vp_x(C);
       else if (lod_probe == 2u) limit = 2.0f * vp_x(C);
-      if (leaf.size <= limit) {
+      if (!C.geom_only && leaf.size <= limit) {
         float t0p = mx(t_enter, tcur - eps_step);
         float t1p = mn(t_leave, t_exit);
         if (t1p >= t0p) {
@@ -462,7 +462,7 @@ static inline ChunkTraceResult trace_chunk_rope_interval_nomacro(const Ctx& C, v
       have_leaf = false;
       continue;
     }
-    if (leaf.mat == MAT_LEAF) {
+    if (leaf.mat == MAT_LEAF && !C.geom_only) {
       float tmin_iter = mx(t_enter, tcur - eps_step);
       LeafCubeHit h2 = leaf_displaced_cube_hit(C, ro, rd, leaf.bmin, leaf.size, time_s, strength, tmin_iter, t_exit);
       if (h2.hit) {
@@ -477,7 +477,7 @@ static inline ChunkTraceResult trace_chunk_rope_interval_nomacro(const Ctx& C, v
     float tmin_iter = mx(t_enter, tcur - eps_step);
     BoxHit bh = cube_hit_normal_from_slab(rd, slab, tmin_iter, t_exit);
     if (bh.hit) {
-      if (leaf.mat == MAT_GRASS) {
+      if (leaf.mat == MAT_GRASS && !C.geom_only) {
         v3 hp = ro + bh.t * rd;
         GrassCell cell = pick_grass_cell_in_chunk_with_bias(hp, bh.n, root_bmin, org, vs, (int32_t)C.cam->chunk_size);
         int32_t cid[3] = {f2i(cell.id_vox.x), f2i(cell.id_vox.y), f2i(cell.id_vox.z)};
@@ -813,6 +813,67 @@ static inline float sun_transmittance_geom_only(const Ctx& C, v3 p, v3 sun_dir) 
     if (cx < gx0 || cy < gy0 || cz < gz0 || cx >= gx1 || cy >= gy1 || cz >= gz1) break;
   }
   return trans;
+}
+
+// ===========================================================================
+// Ray queries (no reference kernel; the reference picks voxels on the CPU,
+// app/app.rs:847-961).  First voxel-geometry hit of a ray within [0, tmax]:
+// a chunk-grid DDA in ray order; in each resident, non-empty chunk the
+// chunk's own AABB interval is traced with the primary pass's macro DDA and
+// rope traversal (trace_chunk_interval_stream_macro) in geometry-only mode.
+// ===========================================================================
+static inline HitGeom query_ray(const Ctx& C, v3 ro, v3 rd, float tmax) {
+  const float vs = vp_x(C);
+  const float csm = (float)C.cam->chunk_size * vs;
+  const int32_t* go = C.cam->grid_origin_chunk;
+  const uint32_t* gd = C.cam->grid_dims;
+  const v3 gbmin = V3((float)go[0], (float)go[1], (float)go[2]) * csm;
+  const v3 gbmax = gbmin + V3((float)gd[0], (float)gd[1], (float)gd[2]) * csm;
+  const v2 rtg = intersect_aabb(ro, rd, gbmin, gbmax);
+  const float t_enter = mx(rtg.x, 0.0f), t_exit = mn(rtg.y, tmax);
+  if (t_exit < t_enter) return miss_hitgeom();
+  const v3 inv = safe_inv3(rd);
+  const v3 p0 = ro + rd * (t_enter + 1e-3f * vs);
+  i3 c = chunk_coord_from_pos(p0, csm);
+  const int32_t sx = rd.x > 0.0f ? 1 : -1, sy = rd.y > 0.0f ? 1 : -1, sz = rd.z > 0.0f ? 1 : -1;
+  float tMaxX = ((rd.x > 0.0f ? (float)(c.x + 1) : (float)c.x) * csm - ro.x) * inv.x;
+  float tMaxY = ((rd.y > 0.0f ? (float)(c.y + 1) : (float)c.y) * csm - ro.y) * inv.y;
+  float tMaxZ = ((rd.z > 0.0f ? (float)(c.z + 1) : (float)c.z) * csm - ro.z) * inv.z;
+  const float tDX = std::fabs(csm * inv.x), tDY = std::fabs(csm * inv.y), tDZ = std::fabs(csm * inv.z);
+  if (std::fabs(rd.x) < EPS_INV) tMaxX = BIG_F32;
+  if (std::fabs(rd.y) < EPS_INV) tMaxY = BIG_F32;
+  if (std::fabs(rd.z) < EPS_INV) tMaxZ = BIG_F32;
+  const int32_t gx1 = go[0] + (int32_t)gd[0], gy1 = go[1] + (int32_t)gd[1], gz1 = go[2] + (int32_t)gd[2];
+  const uint32_t msteps = gd[0] + gd[1] + gd[2] + 2u;
+  float t_cell = t_enter;
+  for (uint32_t s = 0; s < msteps; ++s) {
+    if (t_cell > t_exit) break;
+    if (c.x < go[0] || c.y < go[1] || c.z < go[2] || c.x >= gx1 || c.y >= gy1 || c.z >= gz1) break;
+    const uint32_t slot = grid_lookup_slot(C, c.x, c.y, c.z);
+    if (slot != INVALID_U32 && slot < C.cam->chunk_count) {
+      const vx_chunk_meta ch = ld_chunk(C, slot);
+      if (ch.macro_empty == 0u) {
+        const v3 rb = V3((float)ch.origin[0], (float)ch.origin[1], (float)ch.origin[2]) * vs;
+        const v2 rc = intersect_aabb(ro, rd, rb, rb + V3s(csm));
+        const float a = mx(rc.x, t_enter), b = mn(rc.y, t_exit);
+        if (b >= a) {
+          ChunkTraceResult r = trace_chunk_interval_stream_macro(C, ro, rd, inv, ch, a, b, false, INVALID_U32);
+          if (r.hit.hit != 0u) {
+            r.hit.slot = slot;
+            return r.hit;
+          }
+        }
+      }
+    }
+    if (tMaxX < tMaxY) {
+      if (tMaxX < tMaxZ) { c.x += sx; t_cell = tMaxX; tMaxX += tDX; }
+      else { c.z += sz; t_cell = tMaxZ; tMaxZ += tDZ; }
+    } else {
+      if (tMaxY < tMaxZ) { c.y += sy; t_cell = tMaxY; tMaxY += tDY; }
+      else { c.z += sz; t_cell = tMaxZ; tMaxZ += tDZ; }
+    }
+  }
+  return miss_hitgeom();
 }
 
 // ===========================================================================

@@ -242,4 +242,25 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
   return 0;
 }
 
+// Ray queries: the device query code (begin + one-chunk steps) per ray.
+int vxd_cpu_query(const cpu_scene* scene, const vx_camera* cam, const vx_ray* rays, uint32_t n, vx_hit* hits, int threads) {
+  if (!scene || !cam || (n && (!rays || !hits))) return -1;
+  const Scene S = to_scene(scene);
+  parallel_for(n, std::max(1, threads), [&](uint32_t i, int) {
+    plain::Ctx C;
+    C.S = S;
+    C.cam = cam;
+    C.grass_taper = nullptr;
+    C.geom_only = true;
+    plain::QueryRay q;
+    const vx_ray& r = rays[i];
+    plain::HitGeom h = plain::miss_hitgeom();
+    if (plain::query_begin(C, F3(r.ox, r.oy, r.oz), F3(r.dx, r.dy, r.dz), r.tmax, q))
+      while (!plain::query_step(C, q, h)) {
+      }
+    hits[i] = plain::query_result(h);
+  });
+  return 0;
+}
+
 }  // extern "C"

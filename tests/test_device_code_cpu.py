@@ -52,3 +52,23 @@ def test_device_code_matches_oracle(vx, name, res, frame_ids):
             if mode == "counted":
                 assert [int(c) for c in counts] == [oc.as_dict()[n] for n in vx.COUNT_NAMES]
         assert int(np.sum(ref.hits["kind"] == 1)) > 0
+
+
+def test_device_query_code_matches_oracle(vx):
+    """Ray queries: the device code (begin + chunk steps) against the oracle,
+    bit-exact, on rays of the query workload's kind over the seeded world."""
+    from oracle import oracle as O
+    from voxhip import scenes
+
+    import worlds
+
+    sc, hw = host_world("c1")
+    clip, heights = worlds.empty_clipmap(vx)
+    world = O.World.from_host(hw, clip, heights)
+    cam = vx.camera_build(vx.make_pose((1.0, 1.0, 1.0), 0.0, 0.0, 70.0), (64, 64), (64, 64), 0,
+                          grid_origin=sc.grid_origin, grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
+    rays = scenes.query_rays(np.random.default_rng(3), 20000, sc.grid_origin, sc.grid_dims, tmax_m=51.2)
+    ref = O.query_rays(world, cam, rays, vx.HIT_DTYPE)
+    got = vxd_cpu.query_rays(world, cam, rays, vx.HIT_DTYPE)
+    assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+    assert int(np.sum(ref["kind"] == 1)) > 2000

@@ -1,0 +1,60 @@
+"""Ray queries on the GPU (vx_query_rays): the persistent lane-refill kernel
+against the oracle's query (itself pinned to a dense brute-force walk in
+test_query_oracle.py), bit-exact hit records, under several kernel
+configurations; plus the edge cases of the interface."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import worlds
+from conftest import host_world
+from parity import make_context
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene_cam(gpu, sc, hw):
+    return gpu.camera_build(gpu.make_pose((1.0, 1.0, 1.0), 0.0, 0.0, 70.0), (64, 64), (64, 64), 0,
+                            grid_origin=sc.grid_origin, grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
+
+
+def test_query_matches_oracle(gpu):
+    from oracle import oracle as O
+    from voxhip import scenes
+
+    sc, hw = host_world("c1")
+    ctx = make_context(gpu, hw)
+    cam = _scene_cam(gpu, sc, hw)
+    rays = scenes.query_rays(np.random.default_rng(11), 100_000, sc.grid_origin, sc.grid_dims, tmax_m=51.2)
+    # edge cases: axis-aligned directions, zero range, origin far outside the grid
+    rays[0, 4:7] = (1.0, 0.0, 0.0)
+    rays[1, 4:7] = (0.0, -1.0, 0.0)
+    rays[2, 3] = 0.0
+    rays[3, 0:3] = (-500.0, 10.0, 20.0)
+    rays[3, 4:7] = (1.0, 0.0, 0.0)
+    rays[3, 3] = 1000.0
+    clip, heights = worlds.empty_clipmap(gpu)
+    ref = O.query_rays(O.World.from_host(hw, clip, heights), cam, rays, gpu.HIT_DTYPE)
+    for waves, refill in ((8, 48), (4, 1), (6, 64)):
+        gpu.query_config(ctx, waves_per_simd=waves, refill_below=refill)
+        got = gpu.query_rays(ctx, cam, rays)
+        bad = np.nonzero(got.view(np.uint8).reshape(len(rays), -1).any(axis=1) !=
+                         ref.view(np.uint8).reshape(len(rays), -1).any(axis=1))[0]
+        same = np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+        print(f"[query] waves={waves} refill={refill}: identical={same} hits={int(np.sum(got['kind'] == 1))} "
+              f"ms={gpu.query_timing(ctx):.3f}", flush=True)
+        assert same, (waves, refill, bad[:5])
+    assert int(np.sum(ref["kind"] == 1)) > 10_000
+
+
+def test_query_interface_edges(gpu):
+    sc, hw = host_world("c0")
+    ctx = make_context(gpu, hw)
+    cam = _scene_cam(gpu, sc, hw)
+    d = gpu.DeviceArray(64)
+    gpu.query_rays_device(ctx, cam, d, d, 0)  # n = 0: nothing to do
+    ctx.sync()
+    with pytest.raises(gpu.VoxhipError):
+        gpu.query_config(ctx, waves_per_simd=0, refill_below=99)
+    d.close()

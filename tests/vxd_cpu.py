@@ -45,6 +45,8 @@ def lib():
         _LIB.vxd_cpu_primary.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,
                                          vp]
         _LIB.vxd_cpu_primary.restype = C.c_int
+        _LIB.vxd_cpu_query.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_int]
+        _LIB.vxd_cpu_query.restype = C.c_int
     return _LIB
 
 
@@ -62,3 +64,13 @@ def primary(world, cam, fr, hits_dtype, mode: str = "one", threads: int = 0):
     assert rc == 0
     fr.primary_hist, fr.shadow, fr.hits = hist_out, shadow_out, hits
     return counts
+
+
+def query_rays(world, cam, rays: np.ndarray, hits_dtype, threads: int = 0):
+    """The device query code run on the host: same contract as oracle.query_rays."""
+    threads = threads or min(16, os.cpu_count() or 1)
+    rays = np.ascontiguousarray(rays, np.float32)
+    hits = np.zeros(len(rays), hits_dtype)
+    assert lib().vxd_cpu_query(C.addressof(world.scene), C.addressof(cam), rays.ctypes.data, len(rays),
+                               hits.ctypes.data, threads) == 0
+    return hits

@@ -423,6 +423,65 @@ __global__ __launch_bounds__(256) void k_composite_taa_batch(Scene S, Batch B) {
   plain::composite_taa_pixel(C, fb, x, y);
 }
 
+// ---------------------------------------------------------------------------
+// Ray queries: one persistent work-stealing launch over all rays.  Each lane
+// owns one ray and advances it one chunk per loop trip; lanes whose ray has
+// finished are refilled with new rays (one wavefront-aggregated atomic on the
+// ray cursor) whenever fewer than `refill_below` lanes are still busy, so a
+// wavefront does not idle behind its longest ray.  Workgroups exit when the
+// ray cursor has passed n and none of their lanes holds a ray.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_query_begin(uint32_t* __restrict__ cursor) {
+  if (threadIdx.x == 0) cursor[0] = 0u;
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_query(
+    Scene S, const vx_camera* __restrict__ cam, const vx_ray* __restrict__ rays, vx_hit* __restrict__ hits, uint32_t n,
+    uint32_t* __restrict__ cursor, uint32_t refill_below) {
+  plain::Ctx C;
+  C.S = S;
+  C.cam = cam;
+  C.grass_taper = nullptr;
+  C.geom_only = true;
+  plain::QueryRay q;
+  bool active = false, drained = false;
+  uint32_t id = 0u;
+  const unsigned long long below = (1ull << lane_id()) - 1ull;
+  for (;;) {
+    const uint32_t busy = (uint32_t)__popcll(__ballot(active));
+    if (busy < refill_below) {
+      const bool need = !active && !drained;
+      const unsigned long long m = __ballot(need);
+      if (m != 0ull) {
+        uint32_t base = 0u;
+        if (lane_id() == 0u) base = atomicAdd(cursor, (uint32_t)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (need) {
+          id = base + (uint32_t)__popcll(m & below);
+          if (id < n) {
+            const vx_ray r = rays[id];
+            if (plain::query_begin(C, F3(r.ox, r.oy, r.oz), F3(r.dx, r.dy, r.dz), r.tmax, q)) active = true;
+            else hits[id] = plain::query_result(plain::miss_hitgeom());
+          } else {
+            drained = true;
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0ull) {
+      if (__ballot(!drained) == 0ull) break;
+      continue;
+    }
+    if (active) {
+      plain::HitGeom h;
+      if (plain::query_step(C, q, h)) {
+        hits[id] = plain::query_result(h);
+        active = false;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -490,5 +549,13 @@ hipError_t vx_launch_ws_begin(const vxd::Batch& B, hipStream_t st) {
 }
 hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, hipStream_t st) {
   hipLaunchKernelGGL(k_primary_batch, dim3(persistent), dim3(64), 0, st, S, B);
+  return hipGetLastError();
+}
+
+// Ray queries: reset the cursor, then one persistent launch over all n rays.
+hipError_t vx_launch_query(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,
+                           uint32_t* cursor, uint32_t persistent, uint32_t refill_below, hipStream_t st) {
+  hipLaunchKernelGGL(k_query_begin, dim3(1), dim3(64), 0, st, cursor);
+  hipLaunchKernelGGL(k_query, dim3(persistent), dim3(64), 0, st, S, cam_dev, rays, hits, n, cursor, refill_below);
   return hipGetLastError();
 }

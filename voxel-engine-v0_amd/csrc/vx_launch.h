@@ -25,3 +25,6 @@ hipError_t vx_launch_batch(const vxd::Scene& S, const vxd::Scene& S_godray, cons
 // cursors, then a persistent launch of `persistent` workgroups.
 hipError_t vx_launch_ws_begin(const vxd::Batch& B, hipStream_t st);
 hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, hipStream_t st);
+// Ray queries over n rays (device arrays) as one persistent launch with lane refill.
+hipError_t vx_launch_query(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,
+                           uint32_t* cursor, uint32_t persistent, uint32_t refill_below, hipStream_t st);

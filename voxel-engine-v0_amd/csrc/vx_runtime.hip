@@ -50,6 +50,11 @@ struct vx_ctx {
   uint32_t* grid = nullptr;
   float* clip_height = nullptr;
   vx_clipmap* clip_dev = nullptr;
+  // Ray queries: the scene camera copy, the ray cursor, configuration, timing.
+  vx_camera* qcam_dev = nullptr;
+  uint32_t* qcursor = nullptr;
+  vx_query_config qcfg{8u, 48u, 0u, 0u};
+  hipEvent_t qev[2] = {nullptr, nullptr};
 };
 
 struct vx_view {
@@ -260,6 +265,10 @@ void vx_destroy(vx_ctx* ctx) {
   dfree(ctx->grid);
   dfree(ctx->clip_height);
   dfree(ctx->clip_dev);
+  dfree(ctx->qcam_dev);
+  dfree(ctx->qcursor);
+  for (auto& e : ctx->qev)
+    if (e) hipEventDestroy(e);
   delete ctx;
 }
 
@@ -830,13 +839,73 @@ int vx_sync(vx_ctx* ctx) {
 }
 
 int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n, vx_hit* hits_dev, void* stream) {
-  (void)ctx;
-  (void)scene_cam;
-  (void)rays_dev;
-  (void)n;
-  (void)hits_dev;
-  (void)stream;
-  return vx_fail(VX_E_INVALID, "vx_query_rays: not available in this build");
+  if (!ctx || !scene_cam || (n && (!rays_dev || !hits_dev))) return vx_fail(VX_E_INVALID, "vx_query_rays: null argument");
+  DeviceGuard g(ctx->device);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (!ctx->qcam_dev) {
+    int rc = dalloc(&ctx->qcam_dev, 1);
+    if (!rc) rc = dalloc(&ctx->qcursor, 1);
+    for (int i = 0; i < 2 && !rc; ++i)
+      if (hipEventCreate(&ctx->qev[i]) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "vx_query_rays: hipEventCreate");
+    if (rc) return rc;
+  }
+  VX_HIP(hipMemcpyAsync(ctx->qcam_dev, scene_cam, sizeof(vx_camera), hipMemcpyHostToDevice, st));
+  VX_HIP(hipEventRecord(ctx->qev[0], st));
+  if (n) {
+    const vx_batch_config wave_cfg{ctx->qcfg.waves_per_simd, 1u, 0u, 0u};
+    VX_HIP(vx_launch_query(make_scene(ctx, false), ctx->qcam_dev, rays_dev, hits_dev, n, ctx->qcursor,
+                           persistent_groups(wave_cfg), ctx->qcfg.refill_below, st));
+  }
+  VX_HIP(hipEventRecord(ctx->qev[1], st));
+  return VX_OK;
+}
+
+int vx_query_timing(vx_ctx* ctx, float* ms) {
+  if (!ctx || !ms) return vx_fail(VX_E_INVALID, "vx_query_timing: null argument");
+  if (!ctx->qev[1]) return vx_fail(VX_E_INVALID, "vx_query_timing: no query has run");
+  DeviceGuard g(ctx->device);
+  VX_HIP(hipEventSynchronize(ctx->qev[1]));
+  VX_HIP(hipEventElapsedTime(ms, ctx->qev[0], ctx->qev[1]));
+  return VX_OK;
+}
+
+int vx_query_set_config(vx_ctx* ctx, const vx_query_config* cfg) {
+  if (!ctx || !cfg) return vx_fail(VX_E_INVALID, "vx_query_set_config: null argument");
+  if (cfg->waves_per_simd < 1u || cfg->waves_per_simd > 16u || cfg->refill_below < 1u || cfg->refill_below > 64u)
+    return vx_fail(VX_E_INVALID, "vx_query_set_config: waves_per_simd in 1..16, refill_below in 1..64");
+  ctx->qcfg = *cfg;
+  return VX_OK;
+}
+
+int vx_query_get_config(const vx_ctx* ctx, vx_query_config* cfg) {
+  if (!ctx || !cfg) return vx_fail(VX_E_INVALID, "vx_query_get_config: null argument");
+  *cfg = ctx->qcfg;
+  return VX_OK;
+}
+
+int vx_query_tune(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n, vx_hit* hits_dev,
+                  vx_query_config* chosen) {
+  if (!ctx) return vx_fail(VX_E_INVALID, "vx_query_tune: null context");
+  const uint32_t waves[] = {4u, 6u, 8u};
+  const uint32_t refill[] = {16u, 32u, 48u, 64u};
+  vx_query_config best = ctx->qcfg;
+  float best_ms = 1e30f;
+  for (uint32_t w : waves)
+    for (uint32_t r : refill) {
+      vx_query_config cfg{w, r, 0u, 0u};
+      int rc = vx_query_set_config(ctx, &cfg);
+      float ms = 0.0f;
+      if (!rc) rc = vx_query_rays(ctx, scene_cam, rays_dev, n, hits_dev, nullptr);
+      if (!rc) rc = vx_query_timing(ctx, &ms);
+      if (rc) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = cfg;
+      }
+    }
+  ctx->qcfg = best;
+  if (chosen) *chosen = best;
+  return VX_OK;
 }
 
 }  // extern "C"

@@ -114,6 +114,13 @@ class BatchConfig(C.Structure):
         return {"waves_per_simd": self.waves_per_simd, "grab": self.grab, "use_graph": self.use_graph}
 
 
+class QueryConfig(C.Structure):
+    _fields_ = [("waves_per_simd", C.c_uint32), ("refill_below", C.c_uint32), ("pad", C.c_uint32 * 2)]
+
+    def as_dict(self) -> dict:
+        return {"waves_per_simd": self.waves_per_simd, "refill_below": self.refill_below}
+
+
 class HostWorldView(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("ropes", C.c_void_p), ("meta", C.c_void_p),
                 ("chunk_count", C.c_uint32), ("macro_words", C.c_void_p), ("colinfo_words", C.c_void_p),
@@ -148,6 +155,10 @@ def _load() -> C.CDLL:
         "vx_view_counters": ([vp, C.POINTER(u64), C.c_int], C.c_int),
         "vx_view_timings": ([vp, C.POINTER(f32)], C.c_int),
         "vx_query_rays": ([vp, C.POINTER(Camera), vp, u32, vp, vp], C.c_int),
+        "vx_query_timing": ([vp, C.POINTER(f32)], C.c_int),
+        "vx_query_set_config": ([vp, C.POINTER(QueryConfig)], C.c_int),
+        "vx_query_get_config": ([vp, C.POINTER(QueryConfig)], C.c_int),
+        "vx_query_tune": ([vp, C.POINTER(Camera), vp, u32, vp, C.POINTER(QueryConfig)], C.c_int),
         "vx_sync": ([vp], C.c_int),
         "vx_batch_create": ([vp, C.POINTER(vp), u32, C.POINTER(vp)], C.c_int), "vx_batch_destroy": ([vp], None),
         "vx_batch_render": ([vp, u32, vp], C.c_int), "vx_batch_timing": ([vp, C.POINTER(f32)], C.c_int),
@@ -178,7 +189,7 @@ EXPORTED = (
     "vx_last_error", "vx_version", "vx_device_count", "vx_create", "vx_destroy", "vx_upload_regions", "vx_write_grid",
     "vx_write_clipmap", "vx_view_create", "vx_view_destroy", "vx_view_resize", "vx_view_reset", "vx_write_camera",
     "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings",
-    "vx_query_rays", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
+    "vx_query_rays", "vx_query_timing", "vx_query_set_config", "vx_query_get_config", "vx_query_tune", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
     "vx_batch_set_config", "vx_batch_get_config", "vx_batch_tune", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
     "vx_host_world_build", "vx_host_world_free", "vx_host_world_get", "vx_host_world_edit", "vx_camera_build",
     "vx_clipmap_build",
@@ -447,6 +458,109 @@ class View:
             self.close()
         except Exception:
             pass
+
+
+class DeviceArray:
+    """A device allocation through the HIP runtime (for the C ABI calls that take
+    device pointers, e.g. ray queries); upload / download copy synchronously."""
+
+    _hip = None
+
+    @classmethod
+    def _rt(cls):
+        if cls._hip is None:
+            cls._hip = C.CDLL("libamdhip64.so")
+            cls._hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            cls._hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            cls._hip.hipFree.argtypes = [C.c_void_p]
+        return cls._hip
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        self.owner = True
+        if self._rt().hipMalloc(C.byref(self.ptr), max(self.nbytes, 1)) != 0:
+            raise VoxhipError(f"hipMalloc({self.nbytes}) failed")
+
+    def view(self, offset: int, nbytes: int) -> "DeviceArray":
+        """A non-owning sub-range (the parent must outlive it)."""
+        assert 0 <= offset and offset + nbytes <= self.nbytes
+        v = DeviceArray.__new__(DeviceArray)
+        v.nbytes, v.owner, v.ptr = int(nbytes), False, C.c_void_p(self.ptr.value + int(offset))
+        return v
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if self._rt().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) != 0:  # host to device
+            raise VoxhipError("hipMemcpy to device failed")
+
+    def download(self, out: np.ndarray) -> np.ndarray:
+        assert out.flags.c_contiguous and out.nbytes <= self.nbytes
+        if self._rt().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) != 0:  # device to host
+            raise VoxhipError("hipMemcpy to host failed")
+        return out
+
+    def close(self) -> None:
+        if self.ptr and self.owner:
+            self._rt().hipFree(self.ptr)
+        self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def query_rays_device(ctx: "Context", scene_cam: Camera, rays: DeviceArray, hits: DeviceArray, n: int,
+                      stream: int | None = None) -> None:
+    """vx_query_rays on device arrays (stream-ordered; does not wait)."""
+    _check(LIB.vx_query_rays(ctx._h, C.byref(scene_cam), rays.ptr, n, hits.ptr, stream), "vx_query_rays")
+
+
+def query_rays(ctx: "Context", scene_cam: Camera, rays: np.ndarray) -> np.ndarray:
+    """First voxel hit of each ray ((n, 8) float32 as vx_ray); returns vx_hit records."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = len(rays)
+    d_rays = DeviceArray.from_numpy(rays)
+    d_hits = DeviceArray(n * HIT_DTYPE.itemsize)
+    query_rays_device(ctx, scene_cam, d_rays, d_hits, n)
+    ctx.sync()
+    out = d_hits.download(np.empty(n, HIT_DTYPE))
+    d_rays.close()
+    d_hits.close()
+    return out
+
+
+def query_timing(ctx: "Context") -> float:
+    ms = C.c_float()
+    _check(LIB.vx_query_timing(ctx._h, C.byref(ms)), "vx_query_timing")
+    return float(ms.value)
+
+
+def query_config(ctx: "Context", waves_per_simd: int | None = None, refill_below: int | None = None) -> QueryConfig:
+    """Reads (and, given values, sets) the context's ray-query configuration."""
+    cfg = QueryConfig()
+    _check(LIB.vx_query_get_config(ctx._h, C.byref(cfg)), "vx_query_get_config")
+    if waves_per_simd is not None or refill_below is not None:
+        cfg.waves_per_simd = waves_per_simd or cfg.waves_per_simd
+        cfg.refill_below = refill_below or cfg.refill_below
+        _check(LIB.vx_query_set_config(ctx._h, C.byref(cfg)), "vx_query_set_config")
+    return cfg
+
+
+def query_tune(ctx: "Context", scene_cam: Camera, rays: DeviceArray, hits: DeviceArray, n: int) -> QueryConfig:
+    cfg = QueryConfig()
+    _check(LIB.vx_query_tune(ctx._h, C.byref(scene_cam), rays.ptr, n, hits.ptr, C.byref(cfg)), "vx_query_tune")
+    return cfg
 
 
 class Batch:

@@ -120,3 +120,20 @@ def frame_camera(scene: Scene, world: HostWorld, frame: int, prev_view_proj=None
 def scene_clipmap(scene: Scene, frame: int = 0):
     pose, _ = scene.poses[frame % len(scene.poses)]
     return clipmap_build(scene.params, pose.pos[0], pose.pos[2], levels=scene.clip_levels)
+
+
+def query_rays(rng, n: int, grid_origin, grid_dims, tmax_m: float = 51.2) -> np.ndarray:
+    """Ray-query workload rays: origins uniform in the world AABB, directions
+    uniform on the unit sphere (normalised Gaussian triplets, renormalised in
+    fp32), max distance tmax_m; an (n, 8) float32 array laid out as vx_ray."""
+    lo = np.array(grid_origin, np.float64) * CHUNK_M
+    hi = lo + np.array(grid_dims, np.float64) * CHUNK_M
+    o = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = tmax_m
+    rays[:, 4:7] = d
+    rays[:, 4:7] /= np.linalg.norm(rays[:, 4:7], axis=1, keepdims=True)
+    return rays
