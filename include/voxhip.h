@@ -409,6 +409,80 @@ int vx_camera_build(const vx_pose* pose, uint32_t render_w, uint32_t render_h,
 int vx_clipmap_build(const vx_world_params* p, float cam_x, float cam_z, uint32_t levels,
                      uint32_t res, float base_cell_m, vx_clipmap* out, float* heights);
 
+/* ------------------------------------------------------------------------ */
+/* 3. Serving                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Serving (no reference counterpart: the reference renders one window's view).
+ * A server owns a device thread that runs continuous batching over the
+ * context: every request queued when a launch starts joins it; all pending
+ * view frames render as ONE fused batch (vx_batch_render) while all pending
+ * ray batches run as ONE fused query launch on a second stream, overlapping
+ * on the GPU; requests that arrive meanwhile wait for the next launch.
+ *   * View streams: a client's successive frames (same stream_id) share one
+ *     resident vx_view, so history (TAA, godray) carries over.  A stream's
+ *     resolution is fixed by its first frame.
+ *   * Admission: at submit, a new stream reserves width*height*250 bytes (its
+ *     view) and a ray batch n*64 bytes (rays + hits) of memory_budget; a
+ *     request that does not fit is rejected with VX_E_BUSY.
+ *   * Edits: single-voxel edits rebuild their chunk on host worker threads
+ *     (generator fill + edits + vx_build_chunk); at the next launch boundary
+ *     each rebuilt chunk is written to a fresh node-arena range (best-fit),
+ *     then its macro/colinfo/meta entries are rewritten and the old range is
+ *     freed (double-buffered replacement).  The context's node_capacity must
+ *     leave room for the replacements.
+ * The world must be the one vx_host_world_build packed (slots in grid order)
+ * and already uploaded to the context. */
+typedef struct vx_server vx_server;
+typedef struct vx_server_config {
+    uint64_t memory_budget;          /* bytes for resident view streams + in-flight ray batches */
+    uint32_t max_views_per_launch;   /* default 64 */
+    uint32_t edit_threads;           /* host threads rebuilding edited chunks (default 8) */
+    vx_world_params world;           /* generator of the resident world */
+    int32_t  grid_origin[3];
+    uint32_t grid_dims[3];
+} vx_server_config;
+#define VX_REQ_VIEW 1u
+#define VX_REQ_RAYS 2u
+typedef struct vx_request {
+    uint32_t kind;                   /* VX_REQ_VIEW or VX_REQ_RAYS */
+    uint32_t stream_id;              /* VIEW: the client's view stream */
+    uint32_t width, height;          /* VIEW: render = present resolution */
+    vx_camera camera;                /* VIEW: this frame's camera; RAYS: the scene camera */
+    const vx_ray* rays;              /* RAYS: host array, copied at submit */
+    uint32_t n_rays, pad;
+    float*   out_rgba;               /* VIEW: optional host buffer, width*height RGBA32F final frame */
+    vx_hit*  out_hits;               /* RAYS: optional host buffer of n_rays hits */
+    uint64_t tag;                    /* echoed in the result */
+} vx_request;
+typedef struct vx_result {
+    uint64_t tag;
+    uint32_t kind;
+    int32_t  status;                 /* VX_OK or the launch's error */
+    double   submit_ms, start_ms, done_ms;  /* server clock, ms since vx_server_create */
+} vx_result;
+typedef struct vx_server_stats {
+    uint64_t submitted, admitted, rejected, completed, rays, edits_applied, launches, pad;
+    double   view_p50_ms, view_p99_ms, rays_p50_ms, rays_p99_ms;  /* submit -> done */
+    double   busy_ms, wall_ms;
+    uint64_t memory_in_use, pad2;
+} vx_server_stats;
+int  vx_server_create(vx_ctx* ctx, const vx_host_world_view* world, const vx_server_config* cfg, vx_server** out);
+void vx_server_destroy(vx_server* server);
+/* Queues a request (VX_E_BUSY when admission fails); never blocks on the GPU. */
+int  vx_server_submit(vx_server* server, const vx_request* req);
+/* Next finished request, or VX_E_AGAIN. */
+int  vx_server_poll(vx_server* server, vx_result* out);
+/* Single-voxel edits (world voxel coords, material). */
+int  vx_server_edit(vx_server* server, const int32_t* xyz, const uint8_t* mats, uint32_t n);
+/* Waits until every queued edit is on the device. */
+int  vx_server_wait_edits(vx_server* server);
+/* Waits until every admitted request has finished. */
+int  vx_server_drain(vx_server* server);
+/* Ends a view stream (after its queued frames) and releases its view. */
+int  vx_server_close_stream(vx_server* server, uint32_t stream_id);
+int  vx_server_get_stats(vx_server* server, vx_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

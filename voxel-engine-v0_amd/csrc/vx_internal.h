@@ -24,3 +24,7 @@ static_assert(sizeof(vx_ray) == 32, "vx_ray is 32 bytes");
 
 // Records a message for vx_last_error() on this thread and returns `code`.
 int vx_fail(int code, const char* fmt, ...);
+
+// Context properties for the library's other translation units.
+uint64_t vx_internal_node_capacity(const vx_ctx* ctx);
+int vx_internal_device(const vx_ctx* ctx);

@@ -22,6 +22,9 @@ namespace {
 thread_local char g_err[512] = "";
 }
 
+uint64_t vx_internal_node_capacity(const vx_ctx* ctx);
+int vx_internal_device(const vx_ctx* ctx);
+
 int vx_fail(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -56,6 +59,9 @@ struct vx_ctx {
   vx_query_config qcfg{8u, 48u, 0u, 0u};
   hipEvent_t qev[2] = {nullptr, nullptr};
 };
+
+uint64_t vx_internal_node_capacity(const vx_ctx* ctx) { return ctx->node_cap; }
+int vx_internal_device(const vx_ctx* ctx) { return ctx->device; }
 
 struct vx_view {
   vx_ctx* ctx = nullptr;
@@ -510,7 +516,8 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
   B1.n = 1;
   B1.tiles = ((v->rw + 7u) / 8u) * ((v->rh + 7u) / 8u);
   B1.grab = 1;
-  const uint32_t persistent = persistent_groups(vx_batch_config{8u, 1u, 0u, 0u});
+  // never more persistent workgroups than tiles
+  const uint32_t persistent = std::min(persistent_groups(vx_batch_config{8u, 1u, 0u, 0u}), std::max(B1.tiles, 1u));
   VX_HIP(hipEventRecord(v->ev[0], st));
   if (!count && (mask & VX_PASS_PRIMARY) && !(c->flags & VX_CTX_PRIMARY_STAGED)) VX_HIP(vx_launch_ws_begin(B1, st));
   if (mask & VX_PASS_PRIMARY) {
@@ -748,7 +755,7 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
     VX_HIP(hipStreamSynchronize(st));
   }
   const vxd::Scene S = make_scene(c, false), SG = make_scene(c, true);
-  const uint32_t persistent = persistent_groups(b->cfg);
+  const uint32_t persistent = std::min(persistent_groups(b->cfg), std::max(b->B.tiles, 1u));
   VX_HIP(hipEventRecord(b->ev[0], st));
   if (b->cfg.use_graph) {
     const uint64_t key = ((uint64_t)mask << 32) | ((uint64_t)b->cfg.waves_per_simd << 16) | b->cfg.grab;
@@ -853,8 +860,10 @@ int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_de
   VX_HIP(hipEventRecord(ctx->qev[0], st));
   if (n) {
     const vx_batch_config wave_cfg{ctx->qcfg.waves_per_simd, 1u, 0u, 0u};
-    VX_HIP(vx_launch_query(make_scene(ctx, false), ctx->qcam_dev, rays_dev, hits_dev, n, ctx->qcursor,
-                           persistent_groups(wave_cfg), ctx->qcfg.refill_below, st));
+    // never more persistent workgroups than 64-ray blocks
+    const uint32_t groups = std::min(persistent_groups(wave_cfg), (n + 63u) / 64u);
+    VX_HIP(vx_launch_query(make_scene(ctx, false), ctx->qcam_dev, rays_dev, hits_dev, n, ctx->qcursor, groups,
+                           ctx->qcfg.refill_below, st));
   }
   VX_HIP(hipEventRecord(ctx->qev[1], st));
   return VX_OK;

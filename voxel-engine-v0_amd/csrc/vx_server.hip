@@ -1,0 +1,629 @@
+// vx_server.hip — the serving loop of libvoxhip (vx_server_* in voxhip.h).
+//
+// No reference counterpart: the reference renders one window's view per
+// frame.  This is the continuous-batching front end over the same kernels:
+//   * requests (a frame of a client's view stream, or a batch of ray queries)
+//     are admitted against a device-memory budget at submit time and queued;
+//   * one device thread repeatedly takes everything queued, renders all
+//     pending view frames as ONE fused batch (vx_batch_render) on the
+//     context stream while all pending ray batches run as ONE fused query
+//     launch on a second stream, so the two overlap on the GPU; requests that
+//     arrive meanwhile wait for the next launch;
+//   * each view stream keeps its vx_view (history) resident between frames;
+//   * edits rebuild their chunk on host worker threads (fill + edits + the
+//     chunk builder); at the next launch boundary each rebuilt chunk goes to a
+//     fresh node-arena range (best-fit allocator), then its macro / colinfo /
+//     meta entries are rewritten and the old range is freed: the arena holds
+//     old and new copies until the swap (double-buffered replacement), and no
+//     launch is in flight at the boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "vx_internal.h"
+
+namespace {
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr uint64_t kViewBytesPerPx = 250;  // history + outputs + hits of one view (see vx_view_create)
+constexpr uint64_t kRayBytes = sizeof(vx_ray) + sizeof(vx_hit);
+
+// Best-fit allocator over the node arena (element units).
+class Arena {
+ public:
+  void init(uint64_t cap, const std::vector<std::pair<uint64_t, uint64_t>>& used) {
+    free_.clear();
+    by_size_.clear();
+    std::vector<std::pair<uint64_t, uint64_t>> u = used;
+    std::sort(u.begin(), u.end());
+    uint64_t at = 0;
+    for (auto& [off, len] : u) {
+      if (off > at) add(at, off - at);
+      at = std::max(at, off + len);
+    }
+    if (cap > at) add(at, cap - at);
+  }
+  bool alloc(uint64_t len, uint64_t& off) {
+    if (len == 0) len = 1;
+    auto it = by_size_.lower_bound(len);
+    if (it == by_size_.end()) return false;
+    off = it->second;
+    const uint64_t size = it->first;
+    by_size_.erase(it);
+    free_.erase(off);
+    if (size > len) add(off + len, size - len);
+    return true;
+  }
+  void release(uint64_t off, uint64_t len) {
+    if (len == 0) len = 1;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && next->first == off + len) {  // merge with the following block
+      len += next->second;
+      erase(next->first, next->second);
+    }
+    auto prev = free_.lower_bound(off);
+    if (prev != free_.begin()) {
+      --prev;
+      if (prev->first + prev->second == off) {  // merge with the preceding block
+        off = prev->first;
+        len += prev->second;
+        erase(prev->first, prev->second);
+      }
+    }
+    add(off, len);
+  }
+  uint64_t free_total() const {
+    uint64_t t = 0;
+    for (auto& kv : free_) t += kv.second;
+    return t;
+  }
+
+ private:
+  void add(uint64_t off, uint64_t len) {
+    free_[off] = len;
+    by_size_.emplace(len, off);
+  }
+  void erase(uint64_t off, uint64_t len) {
+    free_.erase(off);
+    auto range = by_size_.equal_range(len);
+    for (auto it = range.first; it != range.second; ++it)
+      if (it->second == off) {
+        by_size_.erase(it);
+        break;
+      }
+  }
+  std::map<uint64_t, uint64_t> free_;            // offset -> length
+  std::multimap<uint64_t, uint64_t> by_size_;    // length -> offset
+};
+
+struct Pending {
+  vx_request req;
+  std::vector<vx_ray> rays;  // copied at submit
+  double submit_ms = 0.0;
+  uint64_t reserved = 0;     // budget bytes released at completion
+};
+
+struct Stream {
+  vx_view* view = nullptr;
+  uint32_t w = 0, h = 0;
+  uint64_t bytes = 0;
+  bool busy = false;  // a frame of this stream is queued or in flight
+};
+
+struct ChunkBuild {
+  uint32_t slot = 0;
+  uint64_t gen = 0;  // the slot's edit generation this build includes
+  std::vector<vx_node> nodes;
+  std::vector<vx_node_ropes> ropes;
+  uint32_t macro[16];
+  uint32_t colinfo[2048];
+};
+
+}  // namespace
+
+struct vx_server {
+  vx_ctx* ctx = nullptr;
+  vx_server_config cfg{};
+  hipStream_t qstream = nullptr;
+  // world
+  std::vector<vx_chunk_meta> meta;  // host mirror of the device meta table
+  Arena arena;
+  std::unordered_map<uint32_t, std::unordered_map<uint32_t, uint8_t>> edits;  // slot -> voxel index -> material
+  std::vector<uint8_t> dirty;        // slot queued for rebuild
+  std::vector<uint64_t> gen;         // per slot: edit generation (bumped by each queued edit batch)
+  std::vector<uint64_t> applied;     // per slot: generation of the build on the device
+  uint32_t building = 0;             // builds taken by workers and not yet in `ready`
+  std::deque<uint32_t> rebuild_q;
+  std::vector<ChunkBuild> ready;     // rebuilt, waiting for a launch boundary
+  std::vector<std::thread> workers;
+  std::condition_variable wcv;
+  // requests
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Pending> inbox;
+  std::deque<vx_result> outbox;
+  std::unordered_map<uint32_t, Stream> streams;
+  uint64_t mem_in_use = 0;
+  bool stop = false;
+  std::thread device;
+  // stats
+  vx_server_stats st{};
+  std::vector<double> lat_view, lat_rays;
+  double t_start = 0.0;
+  int last_error = VX_OK;
+};
+
+namespace {
+
+uint32_t slot_of_voxel(const vx_server* s, int32_t wx, int32_t wy, int32_t wz, uint32_t& idx) {
+  const int32_t cx = (int32_t)std::floor(wx / 64.0), cy = (int32_t)std::floor(wy / 64.0), cz = (int32_t)std::floor(wz / 64.0);
+  const int32_t ix = cx - s->cfg.grid_origin[0], iy = cy - s->cfg.grid_origin[1], iz = cz - s->cfg.grid_origin[2];
+  if (ix < 0 || iy < 0 || iz < 0 || ix >= (int32_t)s->cfg.grid_dims[0] || iy >= (int32_t)s->cfg.grid_dims[1] ||
+      iz >= (int32_t)s->cfg.grid_dims[2])
+    return VX_INVALID_U32;
+  // Slots follow grid order (x fastest), as vx_host_world packs them.
+  const uint32_t slot = ((uint32_t)iz * s->cfg.grid_dims[1] + (uint32_t)iy) * s->cfg.grid_dims[0] + (uint32_t)ix;
+  if (slot >= s->meta.size()) return VX_INVALID_U32;
+  const int32_t lx = wx - cx * 64, ly = wy - cy * 64, lz = wz - cz * 64;
+  idx = (uint32_t)(ly * 4096 + lz * 64 + lx);  // material layout idx = y*4096 + z*64 + x
+  return slot;
+}
+
+// Host worker: rebuild queued chunks (fill, apply the chunk's edits, build).
+void worker_main(vx_server* s) {
+  std::vector<uint8_t> mat(64 * 64 * 64);
+  std::vector<int32_t> ground(64 * 64);
+  ChunkBuild b;
+  int32_t origin[3] = {0, 0, 0};
+  b.nodes.resize(VX_MAX_CHUNK_NODES);
+  b.ropes.resize(VX_MAX_CHUNK_NODES);
+  for (;;) {
+    uint32_t slot;
+    std::vector<std::pair<uint32_t, uint8_t>> ed;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->wcv.wait(lk, [&] { return s->stop || !s->rebuild_q.empty(); });
+      if (s->stop) return;
+      slot = s->rebuild_q.front();
+      s->rebuild_q.pop_front();
+      s->dirty[slot] = 0;  // edits arriving from now on queue the chunk again
+      b.gen = s->gen[slot];
+      ++s->building;
+      for (auto& kv : s->edits[slot]) ed.emplace_back(kv.first, kv.second);
+      origin[0] = s->meta[slot].origin[0];
+      origin[1] = s->meta[slot].origin[1];
+      origin[2] = s->meta[slot].origin[2];
+    }
+    const int32_t cx = origin[0] / 64, cy = origin[1] / 64, cz = origin[2] / 64;
+    uint32_t count = 0;
+    int rc = vx_world_fill_chunk(&s->cfg.world, cx, cy, cz, mat.data(), ground.data());
+    if (!rc) {
+      for (auto& e : ed) mat[e.first] = e.second;
+      rc = vx_build_chunk(mat.data(), ground.data(), cy * 64, s->cfg.world.dirt_depth, b.nodes.data(), b.ropes.data(),
+                          VX_MAX_CHUNK_NODES, &count, b.macro, b.colinfo);
+    }
+    if (rc) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      --s->building;
+      s->last_error = rc;
+      continue;
+    }
+    ChunkBuild out;
+    out.slot = slot;
+    out.gen = b.gen;
+    out.nodes.assign(b.nodes.begin(), b.nodes.begin() + count);
+    out.ropes.assign(b.ropes.begin(), b.ropes.begin() + count);
+    std::memcpy(out.macro, b.macro, sizeof(out.macro));
+    std::memcpy(out.colinfo, b.colinfo, sizeof(out.colinfo));
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->ready.push_back(std::move(out));
+      --s->building;
+    }
+    s->cv.notify_all();
+  }
+}
+
+// Launch boundary: move rebuilt chunks into fresh arena ranges, then swap
+// their meta entries and release the old ranges.  No launch is in flight.
+int apply_ready(vx_server* s, std::vector<ChunkBuild>& builds) {
+  // Newest build of each slot only: builds finish in any order, and an older
+  // one must not replace a newer one.
+  std::sort(builds.begin(), builds.end(), [](const ChunkBuild& a, const ChunkBuild& b) {
+    return a.slot != b.slot ? a.slot < b.slot : a.gen > b.gen;
+  });
+  std::vector<ChunkBuild> latest;
+  for (ChunkBuild& b : builds)
+    if ((latest.empty() || latest.back().slot != b.slot) && b.gen > s->applied[b.slot]) latest.push_back(std::move(b));
+  builds.swap(latest);
+  std::vector<vx_region> rn, rr, rm, rc, mr;
+  std::vector<std::pair<uint64_t, uint64_t>> old;
+  std::vector<std::pair<uint32_t, vx_chunk_meta>> swaps;
+  std::vector<ChunkBuild> later;
+  for (ChunkBuild& b : builds) {
+    uint64_t off = 0;
+    if (!s->arena.alloc(b.nodes.size(), off)) {  // arena full: retry at the next boundary
+      later.push_back(std::move(b));
+      continue;
+    }
+    vx_chunk_meta m = s->meta[b.slot];
+    old.emplace_back(m.node_base, m.node_count);
+    m.node_base = (uint32_t)off;
+    m.node_count = (uint32_t)b.nodes.size();
+    bool any = false;
+    for (uint32_t w : b.macro) any |= w != 0u;
+    m.macro_empty = any ? 0u : 1u;
+    swaps.emplace_back(b.slot, m);
+    rn.push_back(vx_region{off, b.nodes.size(), b.nodes.data()});
+    rr.push_back(vx_region{off, b.ropes.size(), b.ropes.data()});
+    rm.push_back(vx_region{m.macro_base, 16, b.macro});
+    rc.push_back(vx_region{m.colinfo_base, 2048, b.colinfo});
+  }
+  int rc0 = VX_OK;
+  if (!swaps.empty()) {
+    // The new copies first, into ranges no meta entry points at yet; then the
+    // meta swap; then the old ranges are free.
+    if ((rc0 = vx_upload_regions(s->ctx, VX_ARENA_NODES, rn.data(), (uint32_t)rn.size())) ||
+        (rc0 = vx_upload_regions(s->ctx, VX_ARENA_ROPES, rr.data(), (uint32_t)rr.size())) ||
+        (rc0 = vx_upload_regions(s->ctx, VX_ARENA_MACRO, rm.data(), (uint32_t)rm.size())) ||
+        (rc0 = vx_upload_regions(s->ctx, VX_ARENA_COLINFO, rc.data(), (uint32_t)rc.size())))
+      return rc0;
+    for (auto& sw : swaps) {
+      s->meta[sw.first] = sw.second;
+      mr.push_back(vx_region{sw.first, 1, &s->meta[sw.first]});
+    }
+    for (ChunkBuild& b : builds) s->applied[b.slot] = std::max(s->applied[b.slot], b.gen);
+    if ((rc0 = vx_upload_regions(s->ctx, VX_ARENA_META, mr.data(), (uint32_t)mr.size()))) return rc0;
+    for (auto& o : old) s->arena.release(o.first, o.second);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st.edits_applied += swaps.size();
+  }
+  if (!later.empty()) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto& b : later) s->ready.push_back(std::move(b));
+  }
+  return VX_OK;
+}
+
+double pct(std::vector<double> v, double p) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  const size_t i = std::min(v.size() - 1, (size_t)(p / 100.0 * (double)(v.size() - 1) + 0.5));
+  return v[i];
+}
+
+void device_main(vx_server* s) {
+  hipSetDevice(vx_internal_device(s->ctx));
+  std::vector<vx_ray> all_rays;
+  vx_ray* d_rays = nullptr;
+  vx_hit* d_hits = nullptr;
+  size_t d_cap = 0;
+  std::vector<vx_hit> h_hits;
+  for (;;) {
+    std::vector<Pending> views, rays;
+    std::vector<ChunkBuild> builds;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv.wait(lk, [&] { return s->stop || !s->inbox.empty() || !s->ready.empty(); });
+      if (s->stop) break;
+      builds.swap(s->ready);
+      // Everything queued goes into this launch, at most one frame per view
+      // stream and max_views_per_launch views; the rest waits for the next.
+      std::deque<Pending> keep;
+      std::vector<uint32_t> taken;
+      while (!s->inbox.empty()) {
+        Pending p = std::move(s->inbox.front());
+        s->inbox.pop_front();
+        if (p.req.kind == VX_REQ_VIEW) {
+          const bool dup = std::find(taken.begin(), taken.end(), p.req.stream_id) != taken.end();
+          if (dup || views.size() >= s->cfg.max_views_per_launch) {
+            keep.push_back(std::move(p));
+          } else {
+            taken.push_back(p.req.stream_id);
+            views.push_back(std::move(p));
+          }
+        } else {
+          rays.push_back(std::move(p));
+        }
+      }
+      s->inbox.swap(keep);
+    }
+    int rc = builds.empty() ? VX_OK : apply_ready(s, builds);
+    const double t0 = now_ms();
+    // Views: bind each request to its stream's view and write the camera.
+    std::vector<vx_view*> vs;
+    for (Pending& p : views) {
+      Stream* stp = nullptr;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        stp = &s->streams[p.req.stream_id];
+      }
+      if (!stp->view && !rc) rc = vx_view_create(s->ctx, p.req.width, p.req.height, p.req.width, p.req.height, &stp->view);
+      if (!rc) rc = vx_write_camera(stp->view, &p.req.camera);
+      vs.push_back(stp->view);
+    }
+    vx_batch* batch = nullptr;
+    if (!rc && !vs.empty()) {
+      rc = vx_batch_create(s->ctx, vs.data(), (uint32_t)vs.size(), &batch);
+      vx_batch_config bc{6u, 1u, 0u, 0u};  // the view set changes per launch: direct launches
+      if (!rc) rc = vx_batch_set_config(batch, &bc);
+      if (!rc) rc = vx_batch_render(batch, VX_PASS_FRAME, nullptr);
+    }
+    // Ray batches: one fused query launch on the second stream.
+    size_t nr = 0;
+    for (Pending& p : rays) nr += p.rays.size();
+    if (!rc && nr) {
+      all_rays.clear();
+      for (Pending& p : rays) all_rays.insert(all_rays.end(), p.rays.begin(), p.rays.end());
+      if (nr > d_cap) {
+        if (d_rays) hipFree(d_rays);
+        if (d_hits) hipFree(d_hits);
+        d_rays = nullptr;
+        d_hits = nullptr;
+        d_cap = 0;
+        if (hipMalloc((void**)&d_rays, nr * sizeof(vx_ray)) != hipSuccess ||
+            hipMalloc((void**)&d_hits, nr * sizeof(vx_hit)) != hipSuccess)
+          rc = vx_fail(VX_E_NOMEM, "server: ray buffers");
+        else
+          d_cap = nr;
+      }
+      if (!rc && hipMemcpyAsync(d_rays, all_rays.data(), nr * sizeof(vx_ray), hipMemcpyHostToDevice, s->qstream) != hipSuccess)
+        rc = vx_fail(VX_E_DEVICE, "server: ray upload");
+      if (!rc) rc = vx_query_rays(s->ctx, &rays[0].req.camera, d_rays, (uint32_t)nr, d_hits, s->qstream);
+    }
+    if (!rc && batch) rc = vx_sync(s->ctx);
+    if (!rc && nr) {
+      h_hits.resize(nr);
+      if (hipMemcpyAsync(h_hits.data(), d_hits, nr * sizeof(vx_hit), hipMemcpyDeviceToHost, s->qstream) != hipSuccess ||
+          hipStreamSynchronize(s->qstream) != hipSuccess)
+        rc = vx_fail(VX_E_DEVICE, "server: hit download");
+    }
+    // Outputs and results.
+    for (size_t i = 0; i < views.size() && !rc; ++i)
+      if (views[i].req.out_rgba)
+        rc = vx_read_output(vs[i], VX_OUT_OUTPUT, views[i].req.out_rgba,
+                            (size_t)views[i].req.width * views[i].req.height * 16);
+    size_t at = 0;
+    for (Pending& p : rays) {
+      if (!rc && p.req.out_hits) std::memcpy(p.req.out_hits, h_hits.data() + at, p.rays.size() * sizeof(vx_hit));
+      at += p.rays.size();
+    }
+    if (batch) vx_batch_destroy(batch);
+    const double t1 = now_ms();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (rc) s->last_error = rc;
+      ++s->st.launches;
+      s->st.busy_ms += t1 - t0;
+      auto finish = [&](Pending& p, std::vector<double>& lat) {
+        vx_result r{};
+        r.tag = p.req.tag;
+        r.kind = p.req.kind;
+        r.status = rc;
+        r.submit_ms = p.submit_ms - s->t_start;
+        r.start_ms = t0 - s->t_start;
+        r.done_ms = t1 - s->t_start;
+        s->outbox.push_back(r);
+        lat.push_back(t1 - p.submit_ms);
+        s->mem_in_use -= p.reserved;
+        ++s->st.completed;
+      };
+      for (Pending& p : views) {
+        bool more = false;
+        for (auto& q : s->inbox) more |= q.req.kind == VX_REQ_VIEW && q.req.stream_id == p.req.stream_id;
+        s->streams[p.req.stream_id].busy = more;
+        s->st.rays += (uint64_t)p.req.width * p.req.height;
+        finish(p, s->lat_view);
+      }
+      for (Pending& p : rays) {
+        s->st.rays += p.rays.size();
+        finish(p, s->lat_rays);
+      }
+    }
+    s->cv.notify_all();
+  }
+  if (d_rays) hipFree(d_rays);
+  if (d_hits) hipFree(d_hits);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_server_create(vx_ctx* ctx, const vx_host_world_view* world, const vx_server_config* cfg, vx_server** out) {
+  if (!ctx || !world || !cfg || !out) return vx_fail(VX_E_INVALID, "vx_server_create: null argument");
+  *out = nullptr;
+  if (world->chunk_count == 0 || !world->meta) return vx_fail(VX_E_INVALID, "vx_server_create: empty world");
+  vx_server* s = new (std::nothrow) vx_server();
+  if (!s) return vx_fail(VX_E_NOMEM, "vx_server_create: host allocation");
+  s->ctx = ctx;
+  s->cfg = *cfg;
+  if (!s->cfg.max_views_per_launch) s->cfg.max_views_per_launch = 64;
+  if (!s->cfg.edit_threads) s->cfg.edit_threads = 8;
+  s->meta.assign(world->meta, world->meta + world->chunk_count);
+  s->dirty.assign(world->chunk_count, 0);
+  s->gen.assign(world->chunk_count, 0);
+  s->applied.assign(world->chunk_count, 0);
+  std::vector<std::pair<uint64_t, uint64_t>> used;
+  for (const vx_chunk_meta& m : s->meta) used.emplace_back(m.node_base, m.node_count);
+  s->arena.init(vx_internal_node_capacity(ctx), used);  // the node arena is the context's node capacity
+  if (hipSetDevice(vx_internal_device(ctx)) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->qstream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return vx_fail(VX_E_DEVICE, "vx_server_create: query stream");
+  }
+  s->t_start = now_ms();
+  for (uint32_t i = 0; i < s->cfg.edit_threads; ++i) s->workers.emplace_back(worker_main, s);
+  s->device = std::thread(device_main, s);
+  *out = s;
+  return VX_OK;
+}
+
+void vx_server_destroy(vx_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  s->wcv.notify_all();
+  if (s->device.joinable()) s->device.join();
+  for (auto& w : s->workers)
+    if (w.joinable()) w.join();
+  for (auto& kv : s->streams)
+    if (kv.second.view) vx_view_destroy(kv.second.view);
+  if (s->qstream) hipStreamDestroy(s->qstream);
+  delete s;
+}
+
+int vx_server_submit(vx_server* s, const vx_request* req) {
+  if (!s || !req) return vx_fail(VX_E_INVALID, "vx_server_submit: null argument");
+  Pending p;
+  p.req = *req;
+  if (req->kind == VX_REQ_VIEW) {
+    if (!req->width || !req->height || (uint64_t)req->width * req->height > (1ull << 26))
+      return vx_fail(VX_E_INVALID, "vx_server_submit: bad view size");
+  } else if (req->kind == VX_REQ_RAYS) {
+    if (!req->rays && req->n_rays) return vx_fail(VX_E_INVALID, "vx_server_submit: rays missing");
+    p.rays.assign(req->rays, req->rays + req->n_rays);
+  } else {
+    return vx_fail(VX_E_INVALID, "vx_server_submit: bad request kind %u", req->kind);
+  }
+  p.submit_ms = now_ms();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    ++s->st.submitted;
+    uint64_t need = 0;
+    if (req->kind == VX_REQ_VIEW) {
+      Stream& stp = s->streams[req->stream_id];
+      if (stp.w && (stp.w != req->width || stp.h != req->height)) {
+        ++s->st.rejected;
+        return vx_fail(VX_E_INVALID, "vx_server_submit: stream %u is %ux%u", req->stream_id, stp.w, stp.h);
+      }
+      if (!stp.w) need = (uint64_t)req->width * req->height * kViewBytesPerPx;  // a new stream's resident view
+    } else {
+      need = (uint64_t)req->n_rays * kRayBytes;  // in-flight ray buffers
+    }
+    if (s->mem_in_use + need > s->cfg.memory_budget) {
+      ++s->st.rejected;
+      if (req->kind == VX_REQ_VIEW && !s->streams[req->stream_id].w) s->streams.erase(req->stream_id);
+      return vx_fail(VX_E_BUSY, "vx_server_submit: %llu bytes over the %llu-byte budget",
+                     (unsigned long long)(s->mem_in_use + need), (unsigned long long)s->cfg.memory_budget);
+    }
+    s->mem_in_use += need;
+    if (req->kind == VX_REQ_VIEW) {
+      Stream& stp = s->streams[req->stream_id];
+      if (!stp.w) {
+        stp.w = req->width;
+        stp.h = req->height;
+        stp.bytes = need;  // released when the stream closes
+      }
+      stp.busy = true;
+    } else {
+      p.reserved = need;
+    }
+    ++s->st.admitted;
+    s->inbox.push_back(std::move(p));
+  }
+  s->cv.notify_all();
+  return VX_OK;
+}
+
+int vx_server_poll(vx_server* s, vx_result* out) {
+  if (!s || !out) return vx_fail(VX_E_INVALID, "vx_server_poll: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->outbox.empty()) return VX_E_AGAIN;
+  *out = s->outbox.front();
+  s->outbox.pop_front();
+  return VX_OK;
+}
+
+int vx_server_edit(vx_server* s, const int32_t* xyz, const uint8_t* mats, uint32_t n) {
+  if (!s || (n && (!xyz || !mats))) return vx_fail(VX_E_INVALID, "vx_server_edit: null argument");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t idx = 0;
+      const uint32_t slot = slot_of_voxel(s, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], idx);
+      if (slot == VX_INVALID_U32) continue;
+      s->edits[slot][idx] = mats[i];
+      ++s->gen[slot];
+      if (!s->dirty[slot]) {
+        s->dirty[slot] = 1;
+        s->rebuild_q.push_back(slot);
+      }
+    }
+  }
+  s->wcv.notify_all();
+  return VX_OK;
+}
+
+int vx_server_close_stream(vx_server* s, uint32_t stream_id) {
+  if (!s) return vx_fail(VX_E_INVALID, "vx_server_close_stream: null server");
+  vx_view* v = nullptr;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->streams.find(stream_id);
+    if (it == s->streams.end()) return vx_fail(VX_E_INVALID, "vx_server_close_stream: no stream %u", stream_id);
+    // wait until no frame of the stream is queued or in flight
+    s->cv.wait(lk, [&] {
+      for (auto& p : s->inbox)
+        if (p.req.kind == VX_REQ_VIEW && p.req.stream_id == stream_id) return false;
+      return !s->streams[stream_id].busy;
+    });
+    v = it->second.view;
+    s->mem_in_use -= it->second.bytes;
+    s->streams.erase(it);
+  }
+  if (v) vx_view_destroy(v);
+  return VX_OK;
+}
+
+int vx_server_drain(vx_server* s) {
+  if (!s) return vx_fail(VX_E_INVALID, "vx_server_drain: null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv.wait(lk, [&] { return s->inbox.empty() && s->st.completed == s->st.admitted; });
+  return s->last_error;
+}
+
+int vx_server_wait_edits(vx_server* s) {
+  if (!s) return vx_fail(VX_E_INVALID, "vx_server_wait_edits: null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  // every queued edit has been built and applied at a launch boundary
+  s->cv.wait(lk, [&] {
+    if (!s->rebuild_q.empty() || !s->ready.empty() || s->building) return false;
+    for (size_t i = 0; i < s->gen.size(); ++i)
+      if (s->applied[i] < s->gen[i]) return false;
+    return true;
+  });
+  return s->last_error;
+}
+
+int vx_server_get_stats(vx_server* s, vx_server_stats* out) {
+  if (!s || !out) return vx_fail(VX_E_INVALID, "vx_server_get_stats: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->st;
+  out->view_p50_ms = pct(s->lat_view, 50.0);
+  out->view_p99_ms = pct(s->lat_view, 99.0);
+  out->rays_p50_ms = pct(s->lat_rays, 50.0);
+  out->rays_p99_ms = pct(s->lat_rays, 99.0);
+  out->wall_ms = now_ms() - s->t_start;
+  out->memory_in_use = s->mem_in_use;
+  return VX_OK;
+}
+
+}  // extern "C"

@@ -22,6 +22,9 @@ import numpy as np
 LIB_PATH = Path(os.environ.get("VOXHIP_LIB") or (Path(__file__).resolve().parent / "libvoxhip.so"))
 
 VX_OK = 0
+VX_E_BUSY = -6
+VX_E_AGAIN = -7
+REQ_VIEW, REQ_RAYS = 1, 2
 INVALID = 0xFFFFFFFF
 MAT_AIR, MAT_GRASS, MAT_DIRT, MAT_STONE, MAT_WOOD, MAT_LEAF, MAT_LIGHT = range(7)
 MAX_CHUNK_NODES = 299593
@@ -121,6 +124,32 @@ class QueryConfig(C.Structure):
         return {"waves_per_simd": self.waves_per_simd, "refill_below": self.refill_below}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("memory_budget", C.c_uint64), ("max_views_per_launch", C.c_uint32), ("edit_threads", C.c_uint32),
+                ("world", WorldParams), ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_uint32 * 3)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("stream_id", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("camera", Camera), ("rays", C.c_void_p), ("n_rays", C.c_uint32), ("pad", C.c_uint32),
+                ("out_rgba", C.c_void_p), ("out_hits", C.c_void_p), ("tag", C.c_uint64)]
+
+
+class Result(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("kind", C.c_uint32), ("status", C.c_int32), ("submit_ms", C.c_double),
+                ("start_ms", C.c_double), ("done_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "admitted", "rejected", "completed", "rays", "edits_applied",
+                                          "launches", "pad")] + \
+               [(n, C.c_double) for n in ("view_p50_ms", "view_p99_ms", "rays_p50_ms", "rays_p99_ms", "busy_ms",
+                                          "wall_ms")] + [("memory_in_use", C.c_uint64), ("pad2", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if not n.startswith("pad")}
+
+
 class HostWorldView(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("ropes", C.c_void_p), ("meta", C.c_void_p),
                 ("chunk_count", C.c_uint32), ("macro_words", C.c_void_p), ("colinfo_words", C.c_void_p),
@@ -160,6 +189,12 @@ def _load() -> C.CDLL:
         "vx_query_get_config": ([vp, C.POINTER(QueryConfig)], C.c_int),
         "vx_query_tune": ([vp, C.POINTER(Camera), vp, u32, vp, C.POINTER(QueryConfig)], C.c_int),
         "vx_sync": ([vp], C.c_int),
+        "vx_server_create": ([vp, C.POINTER(HostWorldView), C.POINTER(ServerConfig), C.POINTER(vp)], C.c_int),
+        "vx_server_destroy": ([vp], None), "vx_server_submit": ([vp, C.POINTER(Request)], C.c_int),
+        "vx_server_poll": ([vp, C.POINTER(Result)], C.c_int),
+        "vx_server_edit": ([vp, vp, vp, u32], C.c_int), "vx_server_wait_edits": ([vp], C.c_int),
+        "vx_server_drain": ([vp], C.c_int), "vx_server_close_stream": ([vp, u32], C.c_int),
+        "vx_server_get_stats": ([vp, C.POINTER(ServerStats)], C.c_int),
         "vx_batch_create": ([vp, C.POINTER(vp), u32, C.POINTER(vp)], C.c_int), "vx_batch_destroy": ([vp], None),
         "vx_batch_render": ([vp, u32, vp], C.c_int), "vx_batch_timing": ([vp, C.POINTER(f32)], C.c_int),
         "vx_batch_set_config": ([vp, C.POINTER(BatchConfig)], C.c_int),
@@ -189,6 +224,8 @@ EXPORTED = (
     "vx_last_error", "vx_version", "vx_device_count", "vx_create", "vx_destroy", "vx_upload_regions", "vx_write_grid",
     "vx_write_clipmap", "vx_view_create", "vx_view_destroy", "vx_view_resize", "vx_view_reset", "vx_write_camera",
     "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings",
+    "vx_server_create", "vx_server_destroy", "vx_server_submit", "vx_server_poll", "vx_server_edit",
+    "vx_server_wait_edits", "vx_server_drain", "vx_server_close_stream", "vx_server_get_stats",
     "vx_query_rays", "vx_query_timing", "vx_query_set_config", "vx_query_get_config", "vx_query_tune", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
     "vx_batch_set_config", "vx_batch_get_config", "vx_batch_tune", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
     "vx_host_world_build", "vx_host_world_free", "vx_host_world_get", "vx_host_world_edit", "vx_camera_build",
@@ -561,6 +598,96 @@ def query_tune(ctx: "Context", scene_cam: Camera, rays: DeviceArray, hits: Devic
     cfg = QueryConfig()
     _check(LIB.vx_query_tune(ctx._h, C.byref(scene_cam), rays.ptr, n, hits.ptr, C.byref(cfg)), "vx_query_tune")
     return cfg
+
+
+class Server:
+    """The serving loop (vx_server_*): continuous batching of view-stream frames
+    and ray batches over one context, with admission and an edit stream."""
+
+    def __init__(self, ctx: "Context", world: "HostWorld", memory_budget: int, max_views_per_launch: int = 64,
+                 edit_threads: int = 8):
+        cfg = ServerConfig()
+        cfg.memory_budget = memory_budget
+        cfg.max_views_per_launch = max_views_per_launch
+        cfg.edit_threads = edit_threads
+        cfg.world = world.params
+        for i in range(3):
+            cfg.grid_origin[i] = world.grid_origin[i]
+            cfg.grid_dims[i] = world.grid_dims[i]
+        view = HostWorldView()
+        _check(LIB.vx_host_world_get(world._h, C.byref(view)), "vx_host_world_get")
+        h = C.c_void_p()
+        _check(LIB.vx_server_create(ctx._h, C.byref(view), C.byref(cfg), C.byref(h)), "vx_server_create")
+        self._h = h
+        self.ctx, self.world = ctx, world
+        self._keep = {}  # tag -> host buffers the server writes into
+
+    def submit_view(self, stream_id: int, camera: Camera, width: int, height: int, tag: int,
+                    out: np.ndarray | None = None) -> bool:
+        """Queues one frame of a view stream; False when admission rejects it."""
+        r = Request()
+        r.kind, r.stream_id, r.width, r.height, r.camera, r.tag = REQ_VIEW, stream_id, width, height, camera, tag
+        if out is not None:
+            assert out.shape == (height, width, 4) and out.dtype == np.float32 and out.flags.c_contiguous
+            r.out_rgba = out.ctypes.data
+        return self._submit(r, tag, out)
+
+    def submit_rays(self, camera: Camera, rays: np.ndarray, tag: int, out: np.ndarray | None = None) -> bool:
+        rays = np.ascontiguousarray(rays, np.float32)
+        r = Request()
+        r.kind, r.camera, r.rays, r.n_rays, r.tag = REQ_RAYS, camera, rays.ctypes.data, len(rays), tag
+        if out is not None:
+            assert len(out) == len(rays) and out.dtype == HIT_DTYPE and out.flags.c_contiguous
+            r.out_hits = out.ctypes.data
+        return self._submit(r, tag, out)
+
+    def _submit(self, r, tag, out) -> bool:
+        rc = LIB.vx_server_submit(self._h, C.byref(r))
+        if rc == VX_E_BUSY:
+            return False
+        _check(rc, "vx_server_submit")
+        if out is not None:
+            self._keep[tag] = out
+        return True
+
+    def poll(self) -> Result | None:
+        res = Result()
+        rc = LIB.vx_server_poll(self._h, C.byref(res))
+        if rc == VX_E_AGAIN:
+            return None
+        _check(rc, "vx_server_poll")
+        self._keep.pop(res.tag, None)
+        return res
+
+    def edit(self, xyz: np.ndarray, mats: np.ndarray) -> None:
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        mats = np.ascontiguousarray(mats, np.uint8).reshape(-1)
+        _check(LIB.vx_server_edit(self._h, xyz.ctypes.data, mats.ctypes.data, len(mats)), "vx_server_edit")
+
+    def wait_edits(self) -> None:
+        _check(LIB.vx_server_wait_edits(self._h), "vx_server_wait_edits")
+
+    def drain(self) -> None:
+        _check(LIB.vx_server_drain(self._h), "vx_server_drain")
+
+    def close_stream(self, stream_id: int) -> None:
+        _check(LIB.vx_server_close_stream(self._h, stream_id), "vx_server_close_stream")
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(LIB.vx_server_get_stats(self._h, C.byref(st)), "vx_server_get_stats")
+        return st.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:
