@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-1080p", action="store_true", help="skip the 1080p per-view latency run")
     ap.add_argument("--query-batches", type=int, default=256,
                     help="ray batches of the incoherent ray-query workload (0: skip it)")
+    ap.add_argument("--serve-seconds", type=float, default=6.0,
+                    help="duration of the serving workload's arrival schedule (0: skip it)")
     ap.add_argument("--single-batch", action="store_true",
                     help="render the 4K view through a one-view batch (work-stealing launches) instead of vx_render")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline)")
@@ -176,6 +178,7 @@ def main():
     lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup)
     batched = None if a.batch_views <= 0 else batching(vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps)
     queries = None if a.query_batches <= 0 else ray_queries(vx, scenes, a.query_batches)
+    serving = None if a.serve_seconds <= 0 else serve(vx, scenes, a.serve_seconds)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -207,6 +210,7 @@ def main():
         "latency_1080p_ms": lat1080,
         "batched": batched,
         "ray_queries": queries,
+        "serving": serving,
         "setup_s": round(setup_s, 2),
     }
     if counts is not None:
@@ -340,6 +344,94 @@ def ray_queries(vx, scenes, n_batches):
             "fused_gpu_ms": round(gpu_fused_ms, 3), "plain_rays_per_s": round(n / t_plain, 1),
             "speedup": round(t_plain / t_fused, 3), "hit_fraction": round(float(np.mean(hits["kind"] == 1)), 4),
             "tuned": tuned}
+
+
+def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budget=64 << 30):
+    """The serving workload: Poisson arrivals (rate req/s, seed 5) for `seconds`,
+    half render frames of n_streams client view streams (resolutions drawn
+    as in the batching workload, each stream on its own seeded flight, at most
+    one frame pending per stream) and half ray-query batches (log-uniform
+    1e3..1e6 rays), against the 32x4x32-chunk world with a 5-level clipmap,
+    with a concurrent stream of edit_rate random single-voxel edits/s, and
+    admission against a `budget`-byte device memory budget."""
+    import threading
+
+    rng = np.random.default_rng(5)
+    sc = scenes.c4(streams=n_streams, frames=2000)
+    hw = vx.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
+    clip, heights = scenes.scene_clipmap(sc, 0)
+    ctx = vx.Context(node_capacity=int(1.5 * len(hw.nodes)), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
+                     clip_levels=5, clip_res=256)
+    ctx.upload_world(hw)
+    ctx.write_clipmap(clip, heights)
+    sizes = [(320, 180), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
+    res = [sizes[i] for i in rng.integers(0, len(sizes), n_streams)]
+    qcam = vx.camera_build(vx.make_pose((1.0, 1.0, 1.0), 0.0, 0.0), (64, 64), (64, 64), 0, grid_origin=sc.grid_origin,
+                           grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
+    srv = vx.Server(ctx, hw, memory_budget=budget, max_queued_frames=1)
+    pool = scenes.query_rays(rng, 2_000_000, sc.grid_origin, sc.grid_dims, tmax_m=51.2)  # batches are slices of it
+    # arrival schedule
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5)))
+    t_arr = t_arr[t_arr < seconds]
+    kinds = rng.random(len(t_arr)) < 0.5  # True: render frame
+    frame_of = [0] * n_streams
+    stop = threading.Event()
+    ext = sc.grid_dims[0] * 64, sc.grid_dims[2] * 64
+
+    def edits():
+        erng = np.random.default_rng(6)
+        period = 0.02
+        per = max(1, int(edit_rate * period))
+        while not stop.is_set():
+            x = erng.integers(0, ext[0], per)
+            z = erng.integers(0, ext[1], per)
+            y = np.array([vx.ground_height(sc.params, int(a), int(b)) for a, b in zip(x, z)]) + erng.integers(-3, 4, per)
+            srv.edit(np.stack([x, y, z], 1), erng.integers(0, 6, per))
+            stop.wait(period)
+
+    editor = threading.Thread(target=edits, daemon=True)
+    t0 = time.perf_counter()
+    editor.start()
+    submitted = rejected = 0
+    view_rays = ray_rays = 0
+    for i, ta in enumerate(t_arr):
+        dt = t0 + ta - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        if kinds[i]:
+            sid = int(rng.integers(0, n_streams))
+            pose, t_s = sc.stream_poses[sid][frame_of[sid] % len(sc.stream_poses[sid])]
+            w, h = res[sid]
+            cam = vx.camera_build(pose, (w, h), (w, h), frame_of[sid], time_s=t_s, grid_origin=sc.grid_origin,
+                                  grid_dims=sc.grid_dims, chunk_count=hw.chunk_count, jitter=True)
+            ok = srv.submit_view(sid, cam, w, h, tag=i)
+            if ok:
+                frame_of[sid] += 1
+                view_rays += w * h
+        else:
+            k = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
+            off = int(rng.integers(0, len(pool) - k))
+            ok = srv.submit_rays(qcam, pool[off:off + k], tag=i)
+            if ok:
+                ray_rays += k
+        submitted += 1
+        rejected += 0 if ok else 1
+    stop.set()
+    editor.join()
+    srv.drain()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return {"workload": f"Poisson {rate:g} req/s for {seconds:g} s (seed 5), half frames of {n_streams} view streams "
+                        f"(320x180..3840x2160, at most 1 pending frame each), half ray batches (1e3..1e6 rays), "
+                        f"{edit_rate:g} single-voxel edits/s, 32x4x32 chunks + 5-level clipmap, {budget >> 30} GB budget",
+            "requests": submitted, "rejected": rejected, "rejected_rate": round(rejected / max(1, submitted), 4),
+            "completed": int(st["completed"]), "launches": int(st["launches"]),
+            "rays_per_s": round((view_rays + ray_rays) / wall, 1), "view_rays": view_rays, "query_rays": ray_rays,
+            "view_latency_ms": {"p50": round(st["view_p50_ms"], 2), "p99": round(st["view_p99_ms"], 2)},
+            "rays_latency_ms": {"p50": round(st["rays_p50_ms"], 2), "p99": round(st["rays_p99_ms"], 2)},
+            "edits_applied_chunks": int(st["edits_applied"]), "gpu_busy_frac": round(st["busy_ms"] / (wall * 1e3), 3),
+            "wall_s": round(wall, 2)}
 
 
 def cpu_baseline(sc, hw, clip, heights, cam, W, H, rows):

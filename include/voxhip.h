@@ -424,7 +424,8 @@ int vx_clipmap_build(const vx_world_params* p, float cam_x, float cam_z, uint32_
  *     resolution is fixed by its first frame.
  *   * Admission: at submit, a new stream reserves width*height*250 bytes (its
  *     view) and a ray batch n*64 bytes (rays + hits) of memory_budget; a
- *     request that does not fit is rejected with VX_E_BUSY.
+ *     request that does not fit is rejected with VX_E_BUSY, and so is a frame
+ *     of a stream that already has max_queued_frames frames pending.
  *   * Edits: single-voxel edits rebuild their chunk on host worker threads
  *     (generator fill + edits + vx_build_chunk); at the next launch boundary
  *     each rebuilt chunk is written to a fresh node-arena range (best-fit),
@@ -438,6 +439,8 @@ typedef struct vx_server_config {
     uint64_t memory_budget;          /* bytes for resident view streams + in-flight ray batches */
     uint32_t max_views_per_launch;   /* default 64 */
     uint32_t edit_threads;           /* host threads rebuilding edited chunks (default 8) */
+    uint32_t max_queued_frames;      /* per stream, queued + in flight; more is rejected (0: no limit) */
+    uint32_t pad;
     vx_world_params world;           /* generator of the resident world */
     int32_t  grid_origin[3];
     uint32_t grid_dims[3];

@@ -510,6 +510,14 @@ int vx_server_submit(vx_server* s, const vx_request* req) {
     uint64_t need = 0;
     if (req->kind == VX_REQ_VIEW) {
       Stream& stp = s->streams[req->stream_id];
+      if (s->cfg.max_queued_frames && stp.busy) {
+        uint32_t q = 0;
+        for (auto& p2 : s->inbox) q += p2.req.kind == VX_REQ_VIEW && p2.req.stream_id == req->stream_id;
+        if (q + 1u > s->cfg.max_queued_frames) {  // counts the frame in flight too
+          ++s->st.rejected;
+          return vx_fail(VX_E_BUSY, "vx_server_submit: stream %u already has %u frame(s) pending", req->stream_id, q + 1u);
+        }
+      }
       if (stp.w && (stp.w != req->width || stp.h != req->height)) {
         ++s->st.rejected;
         return vx_fail(VX_E_INVALID, "vx_server_submit: stream %u is %ux%u", req->stream_id, stp.w, stp.h);

@@ -126,7 +126,8 @@ class QueryConfig(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_uint64), ("max_views_per_launch", C.c_uint32), ("edit_threads", C.c_uint32),
-                ("world", WorldParams), ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_uint32 * 3)]
+                ("max_queued_frames", C.c_uint32), ("pad", C.c_uint32), ("world", WorldParams),
+                ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_uint32 * 3)]
 
 
 class Request(C.Structure):
@@ -605,11 +606,12 @@ class Server:
     and ray batches over one context, with admission and an edit stream."""
 
     def __init__(self, ctx: "Context", world: "HostWorld", memory_budget: int, max_views_per_launch: int = 64,
-                 edit_threads: int = 8):
+                 edit_threads: int = 8, max_queued_frames: int = 0):
         cfg = ServerConfig()
         cfg.memory_budget = memory_budget
         cfg.max_views_per_launch = max_views_per_launch
         cfg.edit_threads = edit_threads
+        cfg.max_queued_frames = max_queued_frames
         cfg.world = world.params
         for i in range(3):
             cfg.grid_origin[i] = world.grid_origin[i]

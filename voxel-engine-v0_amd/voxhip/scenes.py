@@ -32,6 +32,7 @@ class Scene:
     render: tuple
     clip_levels: int = 5
     poses: list = field(default_factory=list)  # list of (Pose, time_s)
+    stream_poses: list = field(default_factory=list)  # per client stream (serving), lists of (Pose, time_s)
 
 
 def surface_m(params: WorldParams, x_m: float, z_m: float) -> float:
@@ -137,3 +138,15 @@ def query_rays(rng, n: int, grid_origin, grid_dims, tmax_m: float = 51.2) -> np.
     rays[:, 4:7] = d
     rays[:, 4:7] /= np.linalg.norm(rays[:, 4:7], axis=1, keepdims=True)
     return rays
+
+
+def c4(seed: int = 42, streams: int = 32, frames: int = 600) -> Scene:
+    """Serving workload world: the seed-42 generator over 32x4x32 chunks with a
+    5-level clipmap; `streams` client view streams, each on its own seeded
+    camera flight (waypoints seeded by seed + stream index)."""
+    p = world_params(seed)
+    dims = (32, 4, 32)
+    s = Scene("c4", p, (0, 0, 0), dims, (1920, 1080), (1920, 1080), 5)
+    s.poses = flight_poses(p, dims, seed, frames)
+    s.stream_poses = [flight_poses(p, dims, seed + 1 + i, frames) for i in range(streams)]
+    return s
