@@ -53,12 +53,19 @@ __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long
   }
 }
 
+// Occupancy of the primary-pass kernels: 4 waves per SIMD gives the compiler
+// 128 VGPRs.  The pass needs more than that at its peak (the grass sphere
+// trace inside the rope traversal), so it still spills; timed at 3840x2160
+// the persistent kernel runs 30.9 ms at 4 waves, 32.8 at 3, 37.9 at 6 and
+// 39.4 at 2: fewer spills win until latency hiding runs out.
+#define VXD_PRIMARY_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
+
 // ---------------------------------------------------------------------------
 // Primary pass, one kernel: one 8x8 tile per wavefront runs the whole pass per
 // pixel.  COUNT adds per-lane counters of the reference algorithm's loads.
 // ---------------------------------------------------------------------------
 template <bool COUNT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_primary(
+__global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary(
     Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, uint32_t tiles_x, uint32_t ntiles,
     unsigned long long* __restrict__ counters) {
   __shared__ TileLDS L;
@@ -326,7 +333,7 @@ __device__ __forceinline__ bool batch_take(const Batch& B, uint32_t pass, uint32
     const uint32_t lo = band * per, hi = min(B.tiles, lo + per);
     uint32_t got = 0u;
     if (threadIdx.x == 0) got = atomicAdd(B.cursor + pass * kBatchBands + band, B.grab);
-    got = __shfl(got, 0, 64);
+    got = __builtin_amdgcn_readfirstlane(got);
     if (lo + got < hi) {
       t0 = lo + got;
       t1 = min(hi, t0 + B.grab);
@@ -346,7 +353,7 @@ __global__ __launch_bounds__(64) void k_batch_end(Batch B) {
   for (uint32_t v = threadIdx.x; v < B.n; v += 64u) B.ping[v] ^= 1u;
 }
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_primary_batch(Scene S, Batch B) {
+__global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scene S, Batch B) {
   __shared__ TileLDS L;
   if (threadIdx.x < 12u) L.grass_taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
   uint32_t band = blockIdx.x % kBatchBands, steps = 0u, t0 = 0u, t1 = 0u;
