@@ -32,10 +32,36 @@ def gpu(vx):
     return vx
 
 
+def place_lights(sc, hw, n: int = 40, seed: int = 3) -> None:
+    """LIGHT voxels one voxel above the ground at seeded points 2..25 m in
+    front of the scene's first camera pose (the generated terrain has none)."""
+    import math
+
+    import numpy as np
+    from voxhip import ground_height
+    from voxhip.scenes import VOXEL_M
+
+    rng = np.random.default_rng(seed)
+    pose = sc.poses[0][0]
+    fx, fz = math.sin(pose.yaw), math.cos(pose.yaw)
+    xyz = []
+    for _ in range(n):
+        d = float(rng.uniform(2.0, 25.0))
+        s = float(rng.uniform(-0.6, 0.6)) * d
+        x, z = pose.pos[0] + fx * d + fz * s, pose.pos[2] + fz * d - fx * s
+        wx, wz = int(math.floor(x / VOXEL_M)), int(math.floor(z / VOXEL_M))
+        xyz.append((wx, ground_height(sc.params, wx, wz) + 1, wz))
+    hw.edit(np.array(xyz, np.int32), np.full(len(xyz), 6, np.uint8))  # 6 = LIGHT
+
+
 @functools.lru_cache(maxsize=4)
 def host_world(name: str):
+    """Cached worlds: "c0", "c1", and "c0_lit" (c0 with LIGHT voxels in view)."""
     import voxhip
     from voxhip import scenes
 
-    sc = {"c0": scenes.c0, "c1": scenes.c1}[name]()
-    return sc, voxhip.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
+    sc = {"c0": scenes.c0, "c1": scenes.c1, "c0_lit": scenes.c0}[name]()
+    hw = voxhip.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
+    if name == "c0_lit":
+        place_lights(sc, hw)
+    return sc, hw

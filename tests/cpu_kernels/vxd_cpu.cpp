@@ -53,7 +53,20 @@ typedef struct cpu_scene {
 
 namespace {
 
-Scene to_scene(const cpu_scene* s) {
+// The per-slot LIGHT flags the library derives on the device (k_chunk_lights):
+// a slot is flagged when its node range holds a LIGHT leaf.
+std::vector<uint32_t> chunk_lights(const cpu_scene* s) {
+  std::vector<uint32_t> out(s->n_chunks, 0u);
+  for (uint64_t slot = 0; slot < s->n_chunks; ++slot) {
+    const vx_chunk_meta& m = s->chunks[slot];
+    const uint64_t lo = std::min<uint64_t>(m.node_base, s->n_nodes), hi = std::min<uint64_t>(lo + m.node_count, s->n_nodes);
+    for (uint64_t i = lo; i < hi && !out[slot]; ++i)
+      if (s->nodes[i].child_base == LEAF_U32 && s->nodes[i].material == MAT_LIGHT) out[slot] = 1u;
+  }
+  return out;
+}
+
+Scene to_scene(const cpu_scene* s, const std::vector<uint32_t>* lights) {
   Scene S;
   S.chunks = s->chunks;
   S.nodes = reinterpret_cast<const uint4*>(s->nodes);
@@ -69,6 +82,7 @@ Scene to_scene(const cpu_scene* s) {
   S.n_macro = s->n_macro;
   S.n_colinfo = s->n_colinfo;
   S.n_clip = s->n_clip;
+  S.chunk_light = lights ? lights->data() : nullptr;
   return S;
 }
 
@@ -102,7 +116,8 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
                     const float* shadow_in, float* color, float* depth, float* local, float* primary_hist_out,
                     float* shadow_out, vx_hit* hits, int mode, int threads, uint64_t* counts) {
   if (!scene || !cam || !w || !h) return -1;
-  const Scene S = to_scene(scene);
+  const std::vector<uint32_t> lights = chunk_lights(scene);
+  const Scene S = to_scene(scene, &lights);
   FrameBuffers fb{};
   fb.rw = fb.pw = w;
   fb.rh = fb.ph = h;
@@ -245,7 +260,7 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
 // Ray queries: the device query code (begin + one-chunk steps) per ray.
 int vxd_cpu_query(const cpu_scene* scene, const vx_camera* cam, const vx_ray* rays, uint32_t n, vx_hit* hits, int threads) {
   if (!scene || !cam || (n && (!rays || !hits))) return -1;
-  const Scene S = to_scene(scene);
+  const Scene S = to_scene(scene, nullptr);
   parallel_for(n, std::max(1, threads), [&](uint32_t i, int) {
     plain::Ctx C;
     C.S = S;

@@ -32,7 +32,8 @@ def _frames(name, res, frame_ids):
         yield world, cam
 
 
-@pytest.mark.parametrize("name,res,frame_ids", [("c0", (320, 180), [0, 1]), ("c1", (320, 180), [30, 31, 32])])
+@pytest.mark.parametrize("name,res,frame_ids", [("c0", (320, 180), [0, 1]), ("c0_lit", (320, 180), [0, 1]),
+                                               ("c1", (320, 180), [30, 31, 32])])
 def test_device_code_matches_oracle(vx, name, res, frame_ids):
     from oracle import oracle as O
 
@@ -52,6 +53,9 @@ def test_device_code_matches_oracle(vx, name, res, frame_ids):
             if mode == "counted":
                 assert [int(c) for c in counts] == [oc.as_dict()[n] for n in vx.COUNT_NAMES]
         assert int(np.sum(ref.hits["kind"] == 1)) > 0
+        # The lit world exercises the light gather's full march (the others its
+        # no-light shortcut): some pixels then receive local light.
+        assert bool(np.any(ref.local[..., :3] > 0.0)) == (name == "c0_lit")
 
 
 def test_device_query_code_matches_oracle(vx):

@@ -70,6 +70,41 @@ def test_primary_c0_first_frame(gpu, wind, staged):
     assert min(s["kinds"]) > 0
 
 
+def test_light_gather_after_reupload(gpu):
+    """The light gather skips its march where no chunk within reach holds a
+    LIGHT leaf; the per-chunk flags it reads are derived on the device after
+    each upload.  Render the unlit c0 world, then upload the same world with
+    LIGHT voxels placed in view into the same context and render again: both
+    frames must match the oracle, and the second must receive local light."""
+    O = _oracle()
+    from voxhip import scenes
+
+    sc, hw = host_world("c0")
+    _, hw_lit = host_world("c0_lit")
+    clip, heights = scenes.scene_clipmap(sc)
+    ctx = gpu.Context(node_capacity=max(len(hw.nodes), len(hw_lit.nodes)), chunk_capacity=hw.chunk_count,
+                      grid_capacity=len(hw.grid), clip_levels=5, clip_res=256)
+    ctx.write_clipmap(clip, heights)
+    view = gpu.View(ctx, sc.present, sc.render)
+    for name, world in (("unlit", hw), ("lit", hw_lit)):
+        ctx.upload_world(world)
+        cam = scenes.frame_camera(sc, world, 0)
+        view.reset()
+        view.write_camera(cam)
+        view.render(gpu.PASS_PRIMARY | gpu.PASS_WRITE_HITS | gpu.PASS_END_FRAME)
+        w = O.World.from_host(world, clip, heights)
+        fr = O.Frame(*sc.render)
+        O.primary(w, cam, fr, gpu.HIT_DTYPE)
+        s = summary(view.read(gpu.OUT_HITS), fr.hits, view.read(gpu.OUT_DEPTH), fr.depth, view.read(gpu.OUT_COLOR), fr.color)
+        local = view.read(gpu.OUT_LOCAL)
+        s["local_max_abs"] = float(np.max(np.abs(local - fr.local)))
+        s["lit_pixels"] = int(np.sum(np.any(fr.local[..., :3] > 0.0, axis=-1)))
+        _report(f"c0 light gather {name}", s)
+        _assert_parity(s)
+        assert s["local_max_abs"] <= COLOR_ABS_TOL
+        assert (s["lit_pixels"] > 0) == (name == "lit")
+
+
 def test_full_pipeline_multiframe(gpu):
     """Four full frames (primary, local TAA, godray, composite, composite TAA)
     with a moving camera, so history reprojection, anchors and the 1/8 shadow

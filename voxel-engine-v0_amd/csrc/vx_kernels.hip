@@ -559,6 +559,27 @@ hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32
   return hipGetLastError();
 }
 
+// Per chunk slot, whether its node range holds a LIGHT leaf (the flags the
+// local-light gather tests before it marches).  One workgroup per slot; the
+// range [node_base, node_base + node_count) is clipped to the arena.
+__global__ __launch_bounds__(256) void k_chunk_lights(Scene S, uint32_t* __restrict__ out) {
+  const uint32_t slot = blockIdx.x;
+  const vx_chunk_meta& m = S.chunks[slot];
+  const uint64_t lo = min((uint64_t)m.node_base, S.n_nodes), hi = min(lo + (uint64_t)m.node_count, S.n_nodes);
+  int any = 0;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += 256u) {
+    const uint4 n = S.nodes[i];
+    any |= (n.x == LEAF_U32 && n.z == MAT_LIGHT) ? 1 : 0;
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) out[slot] = any ? 1u : 0u;
+}
+
+hipError_t vx_launch_chunk_lights(const vxd::Scene& S, uint32_t* out, uint32_t n_slots, hipStream_t st) {
+  if (n_slots) hipLaunchKernelGGL(k_chunk_lights, dim3(n_slots), dim3(256), 0, st, S, out);
+  return hipGetLastError();
+}
+
 // Ray queries: reset the cursor, then one persistent launch over all n rays.
 hipError_t vx_launch_query(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,
                            uint32_t* cursor, uint32_t persistent, uint32_t refill_below, hipStream_t st) {

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -53,6 +54,10 @@ struct vx_ctx {
   uint32_t* grid = nullptr;
   float* clip_height = nullptr;
   vx_clipmap* clip_dev = nullptr;
+  // Per chunk slot: whether its nodes hold a LIGHT leaf (derived on the device
+  // from the node arena and the chunk table after they change).
+  uint32_t* chunk_light = nullptr;
+  std::atomic<bool> lights_dirty{true};
   // Ray queries: the scene camera copy, the ray cursor, configuration, timing.
   vx_camera* qcam_dev = nullptr;
   uint32_t* qcursor = nullptr;
@@ -138,12 +143,25 @@ vxd::Scene make_scene(const vx_ctx* c, bool godray) {
   S.n_macro = (uint64_t)c->chunk_cap * 16u;
   S.n_colinfo = (uint64_t)c->chunk_cap * 2048u;
   S.n_clip = (uint64_t)c->clip_levels * c->clip_res * c->clip_res;
+  S.chunk_light = c->chunk_light;
   if (godray && !(c->flags & VX_CTX_GODRAY_MACRO_FIX)) {
     // The reference's godray pass sees the rope buffer where it expects macro_occ.
     S.macro = reinterpret_cast<const uint32_t*>(c->ropes);
     S.n_macro = c->node_cap * 8u;
   }
   return S;
+}
+
+// Recomputes the per-slot LIGHT flags on `st` if the node arena or the chunk
+// table changed since they were last derived.
+int refresh_chunk_lights(vx_ctx* c, hipStream_t st) {
+  if (!c->lights_dirty.exchange(false)) return VX_OK;
+  hipError_t e = vx_launch_chunk_lights(make_scene(c, false), c->chunk_light, c->chunk_cap, st);
+  if (e != hipSuccess) {
+    c->lights_dirty = true;
+    return vx_fail(VX_E_DEVICE, "chunk light flags: %s", hipGetErrorString(e));
+  }
+  return VX_OK;
 }
 
 void free_view_buffers(vx_view* v) {
@@ -240,7 +258,7 @@ int vx_create(const vx_caps* caps, vx_ctx** out) {
       (rc = dalloc(&ctx->meta, ctx->chunk_cap)) || (rc = dalloc(&ctx->macro, (size_t)ctx->chunk_cap * 16)) ||
       (rc = dalloc(&ctx->colinfo, (size_t)ctx->chunk_cap * 2048)) || (rc = dalloc(&ctx->grid, ctx->grid_cap)) ||
       (rc = dalloc(&ctx->clip_height, (size_t)ctx->clip_levels * ctx->clip_res * ctx->clip_res)) ||
-      (rc = dalloc(&ctx->clip_dev, 1))) {
+      (rc = dalloc(&ctx->clip_dev, 1)) || (rc = dalloc(&ctx->chunk_light, ctx->chunk_cap))) {
     vx_destroy(ctx);
     return rc;
   }
@@ -271,6 +289,7 @@ void vx_destroy(vx_ctx* ctx) {
   dfree(ctx->grid);
   dfree(ctx->clip_height);
   dfree(ctx->clip_dev);
+  dfree(ctx->chunk_light);
   dfree(ctx->qcam_dev);
   dfree(ctx->qcursor);
   for (auto& e : ctx->qev)
@@ -306,6 +325,7 @@ int vx_upload_regions(vx_ctx* ctx, int arena, const vx_region* regions, uint32_t
     VX_HIP(hipMemcpyAsync(base + r.offset * esz, r.data, r.count * esz, hipMemcpyHostToDevice, ctx->stream));
   }
   VX_HIP(hipStreamSynchronize(ctx->stream));
+  if (arena == VX_ARENA_NODES || arena == VX_ARENA_META) ctx->lights_dirty = true;
   return VX_OK;
 }
 
@@ -518,6 +538,10 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
   B1.grab = 1;
   // never more persistent workgroups than tiles
   const uint32_t persistent = std::min(persistent_groups(vx_batch_config{8u, 1u, 0u, 0u}), std::max(B1.tiles, 1u));
+  if (mask & VX_PASS_PRIMARY) {
+    const int rc = refresh_chunk_lights(c, st);
+    if (rc) return rc;
+  }
   VX_HIP(hipEventRecord(v->ev[0], st));
   if (!count && (mask & VX_PASS_PRIMARY) && !(c->flags & VX_CTX_PRIMARY_STAGED)) VX_HIP(vx_launch_ws_begin(B1, st));
   if (mask & VX_PASS_PRIMARY) {
@@ -756,6 +780,10 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
   }
   const vxd::Scene S = make_scene(c, false), SG = make_scene(c, true);
   const uint32_t persistent = std::min(persistent_groups(b->cfg), std::max(b->B.tiles, 1u));
+  if (mask & VX_PASS_PRIMARY) {
+    const int rc = refresh_chunk_lights(c, st);
+    if (rc) return rc;
+  }
   VX_HIP(hipEventRecord(b->ev[0], st));
   if (b->cfg.use_graph) {
     const uint64_t key = ((uint64_t)mask << 32) | ((uint64_t)b->cfg.waves_per_simd << 16) | b->cfg.grab;

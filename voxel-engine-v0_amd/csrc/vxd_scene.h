@@ -18,6 +18,10 @@ struct Scene {
   const vx_clipmap* __restrict__ clip;
   const float* __restrict__ clip_height;
   uint64_t n_chunks, n_nodes, n_grid, n_macro, n_colinfo, n_clip;
+  // Per chunk slot: nonzero when the slot's nodes hold a LIGHT leaf (derived
+  // from the node arena and the chunk table, never uploaded).  Null: unknown,
+  // so every local-light gather runs in full.
+  const uint32_t* __restrict__ chunk_light;
 };
 
 // Algorithmic loads by array (element counts; bytes = count * element size).
