@@ -57,11 +57,11 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_FILE = REPO / "profiles" / "k_primary_traffic.md"
+TRAFFIC_FILE = REPO / "profiles" / "k_primary_batch_traffic.md"
 
 
 def measured_traffic(W, H):
-    """HBM bytes per k_primary launch at this resolution, from the committed
+    """HBM bytes per k_primary_batch launch at this resolution, from the committed
     counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
     if not TRAFFIC_FILE.exists():
         return None
@@ -156,7 +156,7 @@ def main():
     roofline = None
     counts = None
     if not a.no_count:
-        # Same kernel, counting variant, over the first three timed frames.
+        # The counting form of the pass (the reference algorithm's loads per ray), over the first three timed frames.
         view.reset()
         tot = {n: 0 for n in vx.COUNT_NAMES}
         ncount = min(3, a.steps)
@@ -172,7 +172,7 @@ def main():
         achieved = bytes_per_launch / (prim_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(W, H),
-                    "kernel": "k_primary", "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                    "kernel": "k_primary_batch", "algorithmic_bytes_per_launch": int(bytes_per_launch),
                     "algorithmic_bytes_per_ray": round(bytes_per_launch / rays, 1)}
 
     lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup)
@@ -200,7 +200,7 @@ def main():
         "config": {"workload": f"plain one-view-per-launch full pipeline at {W}x{H}, 16x4x16-chunk world, "
                                f"5-level clipmap, camera flight frames {a.warmup}..{a.warmup + a.steps - 1}",
                    "resolution": [W, H], "chunks": hw.chunk_count, "nodes": int(len(hw.nodes)),
-                   "path": "plain", "primary_kernels": "staged" if a.staged else "one per tile"},
+                   "path": "plain", "primary_kernels": "staged" if a.staged else "persistent work-stealing over the view's 8x8 tiles"},
         "primary_kernel_rays_per_s": round(rays / (prim_ms * 1e-3), 1),
         "pass_ms_mean": {n: round(float(v), 4) for n, v in
                          zip(("primary", "local_taa", "godray", "composite", "composite_taa"), pass_ms.mean(axis=0))},

@@ -146,6 +146,7 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
         plain::tile_build(C, L, wgx, wgy);
         for (uint32_t lane = 0; lane < 64u; ++lane) {
           const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
+          C.grass_blades = L.grass_blades + lane;
           if (gx < w && gy < h) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
         }
       } else {
@@ -157,6 +158,7 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
         counted::tile_build(C, L, wgx, wgy);
         for (uint32_t lane = 0; lane < 64u; ++lane) {
           const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
+          C.grass_blades = L.grass_blades + lane;
           if (gx < w && gy < h) counted::primary_pixel(C, L, fb, gx, gy, counted::sky_bg(F3(0.0f, 1.0f, 0.0f)));
         }
       }
@@ -180,6 +182,7 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
       plain::tile_build(C, L, wgx, wgy);
       for (uint32_t lane = 0; lane < 64u; ++lane) {
         const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
+        C.grass_blades = L.grass_blades + lane;
         if (gx >= w || gy >= h) continue;
         const plain::PrimRay r = plain::prim_ray(C, fb, gx, gy);
         const plain::VoxTraceResult vt = plain::prim_trace(C, L, fb, r);
@@ -197,32 +200,34 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
     std::vector<uint32_t> q[kNumPrimQueues];
     for (uint32_t k = 0; k < kNumPrimQueues; ++k)
       for (auto& part : tq[k]) q[k].insert(q[k].end(), part.begin(), part.end());
-    auto ctx = [&]() {
+    std::vector<float> blade_cache((size_t)threads * kGrassBladeCache);
+    auto ctx = [&](int t) {
       plain::Ctx C;
       C.S = S;
       C.cam = cam;
       C.grass_taper = taper;
+      C.grass_blades = blade_cache.data() + (size_t)t * kGrassBladeCache;
       return C;
     };
     auto ray = [&](const plain::Ctx& C, uint32_t pix) { return plain::prim_ray(C, fb, pix % w, pix / w); };
-    parallel_for((uint32_t)q[kQGrass].size(), threads, [&](uint32_t i, int) {
-      const plain::Ctx C = ctx();
+    parallel_for((uint32_t)q[kQGrass].size(), threads, [&](uint32_t i, int t) {
+      const plain::Ctx C = ctx(t);
       const uint32_t pix = q[kQGrass][i];
       plain::PrimAnchor a;
       plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
       plain::prim_grass_refine(C, ray(C, pix), hg);
       plain::prim_rec_store_tn(rec.data(), pix, hg);
     });
-    parallel_for((uint32_t)q[kQShadow].size(), threads, [&](uint32_t i, int) {
-      const plain::Ctx C = ctx();
+    parallel_for((uint32_t)q[kQShadow].size(), threads, [&](uint32_t i, int t) {
+      const plain::Ctx C = ctx(t);
       const uint32_t pix = q[kQShadow][i];
       plain::PrimAnchor a;
       const plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
       fb.shadow_out[pix] = plain::prim_shadow(C, fb, ray(C, pix), hg);
     });
     std::vector<uint8_t> lit(n, 0);
-    parallel_for((uint32_t)q[kQVox].size(), threads, [&](uint32_t i, int) {
-      const plain::Ctx C = ctx();
+    parallel_for((uint32_t)q[kQVox].size(), threads, [&](uint32_t i, int t) {
+      const plain::Ctx C = ctx(t);
       const uint32_t pix = q[kQVox][i];
       const plain::PrimRay r = ray(C, pix);
       plain::PrimAnchor a;
@@ -233,15 +238,15 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
     });
     for (uint32_t pix : q[kQVox])
       if (lit[pix]) q[kQLight].push_back(pix);
-    parallel_for((uint32_t)q[kQLight].size(), threads, [&](uint32_t i, int) {
-      const plain::Ctx C = ctx();
+    parallel_for((uint32_t)q[kQLight].size(), threads, [&](uint32_t i, int t) {
+      const plain::Ctx C = ctx(t);
       const uint32_t pix = q[kQLight][i];
       plain::PrimAnchor a;
       const plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
       fb.local[pix] = plain::prim_local_light(C, ray(C, pix), hg);
     });
-    parallel_for((uint32_t)q[kQEnv].size(), threads, [&](uint32_t i, int) {
-      const plain::Ctx C = ctx();
+    parallel_for((uint32_t)q[kQEnv].size(), threads, [&](uint32_t i, int t) {
+      const plain::Ctx C = ctx(t);
       const uint32_t pix = q[kQEnv][i];
       plain::prim_env(C, fb, ray(C, pix), plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
     });

@@ -78,6 +78,7 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary(
     C.S = S;
     C.cam = cam;
     C.grass_taper = L.grass_taper;
+    C.grass_blades = L.grass_blades + threadIdx.x;
     Counts cnt = {};
     C.cnt = &cnt;
     if (threadIdx.x == 0) counted::tile_build(C, L, wgx, wgy);
@@ -89,6 +90,7 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary(
     C.S = S;
     C.cam = cam;
     C.grass_taper = L.grass_taper;
+    C.grass_blades = L.grass_blades + threadIdx.x;
     if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
     __syncthreads();
     if (gx < fb.rw && gy < fb.rh) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
@@ -141,6 +143,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
   C.S = S;
   C.cam = cam;
   C.grass_taper = L.grass_taper;
+  C.grass_blades = L.grass_blades + threadIdx.x;
   if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
   __syncthreads();
   const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
@@ -185,12 +188,14 @@ __device__ __forceinline__ plain::Ctx stage_ctx(const Scene& S, const vx_camera*
 
 __global__ __launch_bounds__(64) void k_prim_grass(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, PrimQueues Q) {
   __shared__ float taper[12];
+  __shared__ float blades[kGrassBladeCache];
   if (threadIdx.x < 12u) taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
   __syncthreads();
   uint32_t pix;
   if (!q_item(Q, kQGrass, pix)) return;
   plain::Ctx C = stage_ctx(S, cam);
   C.grass_taper = taper;
+  C.grass_blades = blades + threadIdx.x;
   const plain::PrimRay r = plain::prim_ray(C, fb, pix % fb.rw, pix / fb.rw);
   plain::PrimAnchor a;
   plain::HitGeom h = plain::prim_rec_load(Q.rec, pix, a);
@@ -210,6 +215,7 @@ __global__ __launch_bounds__(64) void k_prim_shadow(Scene S, const vx_camera* __
 
 __global__ __launch_bounds__(64) void k_prim_shade(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, PrimQueues Q) {
   __shared__ float taper[12];
+  __shared__ float blades[kGrassBladeCache];
   if (threadIdx.x < 12u) taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
   __syncthreads();
   const uint32_t n = Q.count[kQVox];
@@ -221,6 +227,7 @@ __global__ __launch_bounds__(64) void k_prim_shade(Scene S, const vx_camera* __r
     pix = Q.q[kQVox][i];
     plain::Ctx C = stage_ctx(S, cam);
     C.grass_taper = taper;
+    C.grass_blades = blades + threadIdx.x;
     const plain::PrimRay r = plain::prim_ray(C, fb, pix % fb.rw, pix / fb.rw);
     plain::PrimAnchor a;
     const plain::HitGeom h = plain::prim_rec_load(Q.rec, pix, a);
@@ -243,12 +250,14 @@ __global__ __launch_bounds__(64) void k_prim_light(Scene S, const vx_camera* __r
 
 __global__ __launch_bounds__(64) void k_prim_env(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, PrimQueues Q) {
   __shared__ float taper[12];
+  __shared__ float blades[kGrassBladeCache];
   if (threadIdx.x < 12u) taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
   __syncthreads();
   uint32_t pix;
   if (!q_item(Q, kQEnv, pix)) return;
   plain::Ctx C = stage_ctx(S, cam);
   C.grass_taper = taper;
+  C.grass_blades = blades + threadIdx.x;
   const plain::PrimRay r = plain::prim_ray(C, fb, pix % fb.rw, pix / fb.rw);
   plain::prim_env(C, fb, r, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
 }
@@ -368,6 +377,7 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
       C.S = S;
       C.cam = V.cam;
       C.grass_taper = L.grass_taper;
+      C.grass_blades = L.grass_blades + threadIdx.x;
       __syncthreads();  // the previous tile's lanes are done with L
       if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
       __syncthreads();
@@ -442,7 +452,7 @@ __global__ __launch_bounds__(64) void k_query_begin(uint32_t* __restrict__ curso
   if (threadIdx.x == 0) cursor[0] = 0u;
 }
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_query(
+__global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_query(
     Scene S, const vx_camera* __restrict__ cam, const vx_ray* __restrict__ rays, vx_hit* __restrict__ hits, uint32_t n,
     uint32_t* __restrict__ cursor, uint32_t refill_below) {
   plain::Ctx C;

@@ -91,12 +91,17 @@ struct Batch {
 
 // Workgroup-shared primary candidate list (common.wgsl:422-424).
 constexpr uint32_t kMaxTileChunks = 192;
+// Blade cache words per workgroup: 3 blades (the most any grass LOD draws) x
+// 11 floats x 64 lanes.  With the tile list this keeps a workgroup's LDS
+// under 10 KiB, so 16 one-wavefront workgroups fit a CU's 160 KiB.
+constexpr uint32_t kGrassBladeCache = 3u * 11u * 64u;
 struct TileLDS {
   uint32_t count;
   uint32_t cached;
   uint32_t slots[kMaxTileChunks];
   float enter[kMaxTileChunks];
   float grass_taper[12];  // blade taper per (lod, segment), see grass_taper_entry
+  float grass_blades[kGrassBladeCache];  // per-lane blade cache, see grass_cache_store
 };
 
 }  // namespace vxd
