@@ -73,6 +73,25 @@ def measured_traffic(W, H):
     return None
 
 
+PROFILE_FILE = REPO / "profiles" / "k_primary_batch_4k_flight.md"
+
+
+def profiled_valu():
+    """VALU busy and lane utilisation of the primary kernel from the committed
+    rocprofv3 summary (SQ counters of the same 4K workload), or None."""
+    if not PROFILE_FILE.exists():
+        return None
+    out = {}
+    for line in PROFILE_FILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 2 and cells[0] in ("VALU busy", "VALU lane utilisation", "L2 hit rate"):
+            out[cells[0]] = round(float(cells[1].rstrip(" %")) / 100.0, 4)
+    if len(out) < 3:
+        return None
+    return {"valu_busy": out["VALU busy"], "valu_lane_utilisation": out["VALU lane utilisation"],
+            "l2_hit_rate": out["L2 hit rate"], "source": str(PROFILE_FILE.relative_to(REPO))}
+
+
 def copy_calibration(mib):
     """Three device-to-device copies of MIB MiB through the HIP runtime's own
     copy kernel: the reference workload for the FETCH_SIZE / WRITE_SIZE scale."""
@@ -173,7 +192,7 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(W, H),
                     "kernel": "k_primary_batch", "algorithmic_bytes_per_launch": int(bytes_per_launch),
-                    "algorithmic_bytes_per_ray": round(bytes_per_launch / rays, 1)}
+                    "algorithmic_bytes_per_ray": round(bytes_per_launch / rays, 1), "profiled": profiled_valu()}
 
     lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup)
     batched = None if a.batch_views <= 0 else batching(vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps)
