@@ -262,6 +262,15 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
   return 0;
 }
 
+// The device float->int conversions, elementwise.
+int vxd_cpu_convert(const float* x, uint32_t n, int32_t* out_i, uint32_t* out_u) {
+  for (uint32_t k = 0; k < n; ++k) {
+    out_i[k] = f2i(x[k]);
+    out_u[k] = f2u(x[k]);
+  }
+  return 0;
+}
+
 // Ray queries: the device query code (begin + one-chunk steps) per ray.
 int vxd_cpu_query(const cpu_scene* scene, const vx_camera* cam, const vx_ray* rays, uint32_t n, vx_hit* hits, int threads) {
   if (!scene || !cam || (n && (!rays || !hits))) return -1;

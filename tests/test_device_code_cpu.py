@@ -76,3 +76,26 @@ def test_device_query_code_matches_oracle(vx):
     got = vxd_cpu.query_rays(world, cam, rays, vx.HIT_DTYPE)
     assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
     assert int(np.sum(ref["kind"] == 1)) > 2000
+
+
+def test_device_float_to_int_conversions():
+    """The device code's f32 -> i32/u32 conversions follow WGSL's saturating
+    rule (NaN -> 0, clamp to the integer range, else truncate toward zero),
+    written here from the rule itself in float64, on edge values and a sweep."""
+    specials = [np.nan, -np.nan, np.inf, -np.inf, 0.0, -0.0, 0.5, -0.5, 0.99999994, -0.99999994, 1.0, -1.0,
+                2147483520.0, 2147483648.0, -2147483648.0, -2147483904.0, 4294967040.0, 4294967296.0,
+                16777216.0, -16777217.0, 1e-45, -1e-45, 3.4e38, -3.4e38]
+    rng = np.random.default_rng(5)
+    sweep = np.concatenate([rng.uniform(-3e9, 5e9, 20000), rng.uniform(-300.0, 300.0, 20000),
+                            np.ldexp(rng.uniform(-1.0, 1.0, 20000), rng.integers(-30, 34, 20000))])
+    x = np.concatenate([np.array(specials), sweep]).astype(np.float32)
+    got_i, got_u = vxd_cpu.convert(x)
+    xd = x.astype(np.float64)
+    i_max = float(np.float32(2147483520.0))
+    u_max = float(np.float32(4294967040.0))
+    want_i = np.where(np.isnan(xd), 0, np.where(xd >= i_max, 2**31 - 1,
+                      np.where(xd <= -2.0**31, -2**31, np.trunc(np.nan_to_num(xd, posinf=0.0, neginf=0.0)))))
+    want_u = np.where(np.isnan(xd) | (xd <= 0.0), 0, np.where(xd >= u_max, 2**32 - 1,
+                      np.trunc(np.nan_to_num(xd, posinf=0.0, neginf=0.0))))
+    assert np.array_equal(got_i.astype(np.int64), want_i.astype(np.int64))
+    assert np.array_equal(got_u.astype(np.int64), want_u.astype(np.int64))

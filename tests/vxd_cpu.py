@@ -47,6 +47,8 @@ def lib():
         _LIB.vxd_cpu_primary.restype = C.c_int
         _LIB.vxd_cpu_query.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_int]
         _LIB.vxd_cpu_query.restype = C.c_int
+        _LIB.vxd_cpu_convert.argtypes = [vp, C.c_uint32, vp, vp]
+        _LIB.vxd_cpu_convert.restype = C.c_int
     return _LIB
 
 
@@ -74,3 +76,12 @@ def query_rays(world, cam, rays: np.ndarray, hits_dtype, threads: int = 0):
     assert lib().vxd_cpu_query(C.addressof(world.scene), C.addressof(cam), rays.ctypes.data, len(rays),
                                hits.ctypes.data, threads) == 0
     return hits
+
+
+def convert(x: np.ndarray):
+    """The device code's saturating f32 -> i32 and f32 -> u32 conversions."""
+    x = np.ascontiguousarray(x, np.float32)
+    oi = np.zeros(len(x), np.int32)
+    ou = np.zeros(len(x), np.uint32)
+    assert lib().vxd_cpu_convert(x.ctypes.data, len(x), oi.ctypes.data, ou.ctypes.data) == 0
+    return oi, ou

@@ -82,17 +82,23 @@ VXD float smoothstepf(float lo, float hi, float x) {
 VXD float signf(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
 VXD int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Saturating float->int (WGSL conversion semantics; NaN -> 0).
+// Saturating float->int (WGSL conversion semantics; NaN -> 0): values at or
+// above 2147483520 (the largest float below 2^31) give INT32_MAX, values at
+// or below -2^31 give INT32_MIN, others truncate.  Written as clamp, convert
+// and selects: the early-return form compiles to nested branches.
 VXD int32_t f2i(float x) {
-  if (!(x == x)) return 0;
-  if (x >= 2147483520.0f) return 2147483647;
-  if (x <= -2147483648.0f) return (int32_t)0x80000000;
-  return (int32_t)x;
+  const float xc = fminf(fmaxf(x, -2147483648.0f), 2147483520.0f);  // in range (or NaN, replaced below)
+  int32_t r = (int32_t)xc;
+  r = x >= 2147483520.0f ? 2147483647 : r;
+  return x == x ? r : 0;
 }
+// Saturating float->uint: NaN and values <= 0 give 0, values at or above
+// 4294967040 (the largest float below 2^32) give UINT32_MAX, others truncate.
 VXD uint32_t f2u(float x) {
-  if (!(x == x) || x <= 0.0f) return 0u;
-  if (x >= 4294967040.0f) return 0xFFFFFFFFu;
-  return (uint32_t)x;
+  const float xc = fminf(fmaxf(x, 0.0f), 4294967040.0f);
+  uint32_t r = (uint32_t)xc;
+  r = x >= 4294967040.0f ? 0xFFFFFFFFu : r;
+  return (x == x && x > 0.0f) ? r : 0u;
 }
 VXD uint32_t fbits(float f) { return __float_as_uint(f); }
 VXD float from_bits(uint32_t u) { return __uint_as_float(u); }
