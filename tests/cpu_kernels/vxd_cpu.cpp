@@ -271,6 +271,18 @@ int vxd_cpu_convert(const float* x, uint32_t n, int32_t* out_i, uint32_t* out_u)
   return 0;
 }
 
+// The device transcendentals, elementwise: out[5n] = sin, cos, exp, log, pow(x, y).
+int vxd_cpu_math(const float* x, const float* y, uint32_t n, float* out) {
+  for (uint32_t k = 0; k < n; ++k) {
+    out[5 * k + 0] = vx_sin(x[k]);
+    out[5 * k + 1] = vx_cos(x[k]);
+    out[5 * k + 2] = vx_exp(x[k]);
+    out[5 * k + 3] = vx_log(x[k]);
+    out[5 * k + 4] = vx_pow(x[k], y[k]);
+  }
+  return 0;
+}
+
 // Ray queries: the device query code (begin + one-chunk steps) per ray.
 int vxd_cpu_query(const cpu_scene* scene, const vx_camera* cam, const vx_ray* rays, uint32_t n, vx_hit* hits, int threads) {
   if (!scene || !cam || (n && (!rays || !hits))) return -1;

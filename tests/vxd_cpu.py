@@ -49,6 +49,8 @@ def lib():
         _LIB.vxd_cpu_query.restype = C.c_int
         _LIB.vxd_cpu_convert.argtypes = [vp, C.c_uint32, vp, vp]
         _LIB.vxd_cpu_convert.restype = C.c_int
+        _LIB.vxd_cpu_math.argtypes = [vp, vp, C.c_uint32, vp]
+        _LIB.vxd_cpu_math.restype = C.c_int
     return _LIB
 
 
@@ -85,3 +87,12 @@ def convert(x: np.ndarray):
     ou = np.zeros(len(x), np.uint32)
     assert lib().vxd_cpu_convert(x.ctypes.data, len(x), oi.ctypes.data, ou.ctypes.data) == 0
     return oi, ou
+
+
+def math(x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """The device code's sin, cos, exp, log and pow(x, y), as an (n, 5) array."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    out = np.zeros((len(x), 5), np.float32)
+    assert lib().vxd_cpu_math(x.ctypes.data, y.ctypes.data, len(x), out.ctypes.data) == 0
+    return out

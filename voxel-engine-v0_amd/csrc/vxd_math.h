@@ -108,10 +108,11 @@ VXD float ldexp_i(float x, int k) {
   int k1 = k / 2, k2 = k - k1;
   return x * from_bits((uint32_t)(127 + k1) << 23) * from_bits((uint32_t)(127 + k2) << 23);
 }
-VXD float vx_exp(float x) {
-  if (!(x == x)) return x;
-  if (x > 88.72f) return __builtin_inff();
-  if (x < -103.9f) return 0.0f;
+// Special cases (NaN, overflow, underflow) are selected after the main path,
+// which runs on x clamped into range: the same value for every x in range,
+// without a branch around the polynomial.
+VXD float vx_exp(float x_in) {
+  const float x = fminf(fmaxf(x_in, -103.9f), 88.72f);
   const float ln2_hi = 0.693145751953125f;
   const float ln2_lo = 1.428606765330187045e-06f;
   float kf = floorf(x * 1.44269504088896341f + 0.5f);
@@ -126,25 +127,21 @@ VXD float vx_exp(float x) {
   p = p * r + 0.5f;
   p = p * r + 1.0f;
   p = p * r + 1.0f;
-  return ldexp_i(p, (int)kf);
+  float v = ldexp_i(p, (int)kf);
+  v = x_in > 88.72f ? __builtin_inff() : v;
+  v = x_in < -103.9f ? 0.0f : v;
+  return x_in == x_in ? v : x_in;
 }
+// Special cases (NaN, negative, zero, infinity) are selected at the end; the
+// subnormal rescale and the mantissa fold are selects too.
 VXD float vx_log(float x) {
-  if (!(x == x) || x < 0.0f) return __builtin_nanf("");
-  if (x == 0.0f) return -__builtin_inff();
-  if (x == __builtin_inff()) return x;
-  int e = 0;
-  uint32_t u = fbits(x);
-  if ((u >> 23) == 0) {
-    x = x * 16777216.0f;
-    u = fbits(x);
-    e = -24;
-  }
-  e += (int)((u >> 23) & 0xFFu) - 127;
+  const bool sub = (fbits(x) >> 23) == 0u;  // +0 or subnormal
+  const uint32_t u = fbits(sub ? x * 16777216.0f : x);
+  int e = (sub ? -24 : 0) + (int)((u >> 23) & 0xFFu) - 127;
   float m = from_bits((u & 0x007FFFFFu) | 0x3F800000u);
-  if (m > 1.41421356f) {
-    m = m * 0.5f;
-    e += 1;
-  }
+  const bool fold = m > 1.41421356f;
+  m = fold ? m * 0.5f : m;
+  e += fold ? 1 : 0;
   float s = (m - 1.0f) / (m + 1.0f);
   float s2 = s * s;
   float q = 2.0f / 13.0f;
@@ -157,7 +154,10 @@ VXD float vx_log(float x) {
   const float ln2_hi = 0.693145751953125f;
   const float ln2_lo = 1.428606765330187045e-06f;
   float ef = (float)e;
-  return (ef * ln2_hi + logm) + ef * ln2_lo;
+  float v = (ef * ln2_hi + logm) + ef * ln2_lo;
+  v = x == __builtin_inff() ? x : v;
+  v = x == 0.0f ? -__builtin_inff() : v;
+  return (!(x == x) || x < 0.0f) ? __builtin_nanf("") : v;
 }
 VXD float vx_pow(float x, float y) {
   if (y == 0.0f) return 1.0f;
@@ -175,7 +175,10 @@ VXD int vx_trig_reduce(float x, float* r) {
   const float pio2_3 = 7.549789948768648e-08f;
   float kf = floorf(x * two_over_pi + 0.5f);
   *r = ((x - kf * pio2_1) - kf * pio2_2) - kf * pio2_3;
-  return (int)(((long long)kf) & 3);
+  // kf mod 4 as two's complement.  |kf| >= 2^31 is a multiple of 256, so its
+  // low bits are 0 (the value a 64-bit conversion gives too); below that the
+  // 32-bit conversion is exact.  Avoids an emulated float -> int64 convert.
+  return (fabsf(kf) < 2147483648.0f ? (int)kf : 0) & 3;
 }
 VXD float vx_sin_poly(float r) {
   float r2 = r * r;
@@ -196,21 +199,22 @@ VXD float vx_cos_poly(float r) {
   p = p * r2 - 0.5f;
   return 1.0f + r2 * p;
 }
+// NaN and infinities give NaN (selected after the main path).
 VXD float vx_sin(float x) {
-  if (!(x == x) || fabsf(x) == __builtin_inff()) return __builtin_nanf("");
   float r;
   int q = vx_trig_reduce(x, &r);
   float s = vx_sin_poly(r), c = vx_cos_poly(r);
   float v = (q & 1) ? c : s;
-  return (q & 2) ? -v : v;
+  v = (q & 2) ? -v : v;
+  return (x == x && fabsf(x) != __builtin_inff()) ? v : __builtin_nanf("");
 }
 VXD float vx_cos(float x) {
-  if (!(x == x) || fabsf(x) == __builtin_inff()) return __builtin_nanf("");
   float r;
   int q = vx_trig_reduce(x, &r);
   float s = vx_sin_poly(r), c = vx_cos_poly(r);
   float v = (q & 1) ? s : c;
-  return ((q + 1) & 2) ? -v : v;
+  v = ((q + 1) & 2) ? -v : v;
+  return (x == x && fabsf(x) != __builtin_inff()) ? v : __builtin_nanf("");
 }
 
 }  // namespace vxd
