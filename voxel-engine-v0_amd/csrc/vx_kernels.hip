@@ -59,6 +59,10 @@ __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long
 // the persistent kernel runs 30.9 ms at 4 waves, 32.8 at 3, 37.9 at 6 and
 // 39.4 at 2: fewer spills win until latency hiding runs out.
 #define VXD_PRIMARY_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
+// The godray kernels: 6 waves per SIMD (80 VGPRs, a few spills) beat 4
+// (120 VGPRs, none) at 3840x2160, 4.27 against 4.51 ms: their sun-shadow
+// traversals are latency-bound.
+#define VXD_GODRAY_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6, 8)))
 
 // ---------------------------------------------------------------------------
 // Primary pass, one kernel: one 8x8 tile per wavefront runs the whole pass per
@@ -268,7 +272,7 @@ __global__ __launch_bounds__(256) void k_local_taa(FrameBuffers fb) {
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(64) void k_godray(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, uint32_t tiles_x,
+__global__ __launch_bounds__(64) VXD_GODRAY_OCCUPANCY void k_godray(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, uint32_t tiles_x,
                                                 uint32_t ntiles, unsigned long long* __restrict__ counters) {
   const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
   const uint32_t gx = (tile % tiles_x) * 8u + (threadIdx.x & 7u), gy = (tile / tiles_x) * 8u + (threadIdx.x >> 3);
@@ -389,7 +393,7 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
 
 // Godray: one workgroup per tile of every view (a flattened grid; the pass is
 // short and even, so plain dispatch beats work stealing), XCD bands as in k_godray.
-__global__ __launch_bounds__(64) void k_godray_batch(Scene S, Batch B) {
+__global__ __launch_bounds__(64) VXD_GODRAY_OCCUPANCY void k_godray_batch(Scene S, Batch B) {
   const uint32_t t = xcd_tile(blockIdx.x, B.tiles);
   const uint32_t v = batch_view_of(B, t, 0);
   const BatchView& V = B.views[v];
