@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-1080p", action="store_true", help="skip the 1080p per-view latency run")
     ap.add_argument("--query-batches", type=int, default=256,
                     help="ray batches of the incoherent ray-query workload (0: skip it)")
+    ap.add_argument("--serve-target-ms", type=float, default=100.0,
+                    help="the serving leg's latency target (admission and launch size); 0 runs without one")
     ap.add_argument("--serve-seconds", type=float, default=6.0,
                     help="duration of the serving workload's arrival schedule (0: skip it)")
     ap.add_argument("--single-batch", action="store_true",
@@ -197,7 +199,10 @@ def main():
     lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup)
     batched = None if a.batch_views <= 0 else batching(vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps)
     queries = None if a.query_batches <= 0 else ray_queries(vx, scenes, a.query_batches)
-    serving = None if a.serve_seconds <= 0 else serve(vx, scenes, a.serve_seconds)
+    serving = None if a.serve_seconds <= 0 else serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms)
+    serving_untargeted = None
+    if a.serve_seconds > 0 and a.serve_target_ms > 0:
+        serving_untargeted = serve(vx, scenes, a.serve_seconds, target_ms=0.0)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -230,6 +235,7 @@ def main():
         "batched": batched,
         "ray_queries": queries,
         "serving": serving,
+        "serving_without_latency_target": serving_untargeted,
         "setup_s": round(setup_s, 2),
     }
     if counts is not None:
@@ -365,14 +371,16 @@ def ray_queries(vx, scenes, n_batches):
             "tuned": tuned}
 
 
-def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budget=64 << 30):
+def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budget=64 << 30, target_ms=100.0):
     """The serving workload: Poisson arrivals (rate req/s, seed 5) for `seconds`,
     half render frames of n_streams client view streams (resolutions drawn
     as in the batching workload, each stream on its own seeded flight, at most
     one frame pending per stream) and half ray-query batches (log-uniform
     1e3..1e6 rays), against the 32x4x32-chunk world with a 5-level clipmap,
     with a concurrent stream of edit_rate random single-voxel edits/s, and
-    admission against a `budget`-byte device memory budget."""
+    admission against a `budget`-byte device memory budget and, when
+    target_ms > 0, against that latency target (the server also bounds each
+    launch to half the target's worth of rays)."""
     import threading
 
     rng = np.random.default_rng(5)
@@ -387,7 +395,7 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
     res = [sizes[i] for i in rng.integers(0, len(sizes), n_streams)]
     qcam = vx.camera_build(vx.make_pose((1.0, 1.0, 1.0), 0.0, 0.0), (64, 64), (64, 64), 0, grid_origin=sc.grid_origin,
                            grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
-    srv = vx.Server(ctx, hw, memory_budget=budget, max_queued_frames=1)
+    srv = vx.Server(ctx, hw, memory_budget=budget, max_queued_frames=1, latency_target_ms=target_ms)
     pool = scenes.query_rays(rng, 2_000_000, sc.grid_origin, sc.grid_dims, tmax_m=51.2)  # batches are slices of it
     # arrival schedule
     t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5)))
@@ -443,7 +451,8 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
     srv.close()
     return {"workload": f"Poisson {rate:g} req/s for {seconds:g} s (seed 5), half frames of {n_streams} view streams "
                         f"(320x180..3840x2160, at most 1 pending frame each), half ray batches (1e3..1e6 rays), "
-                        f"{edit_rate:g} single-voxel edits/s, 32x4x32 chunks + 5-level clipmap, {budget >> 30} GB budget",
+                        f"{edit_rate:g} single-voxel edits/s, 32x4x32 chunks + 5-level clipmap, {budget >> 30} GB budget, "
+                        + (f"{target_ms:g} ms latency target" if target_ms > 0 else "no latency target"),
             "requests": submitted, "rejected": rejected, "rejected_rate": round(rejected / max(1, submitted), 4),
             "completed": int(st["completed"]), "launches": int(st["launches"]),
             "rays_per_s": round((view_rays + ray_rays) / wall, 1), "view_rays": view_rays, "query_rays": ray_rays,

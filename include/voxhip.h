@@ -426,6 +426,13 @@ int vx_clipmap_build(const vx_world_params* p, float cam_x, float cam_z, uint32_
  *     view) and a ray batch n*64 bytes (rays + hits) of memory_budget; a
  *     request that does not fit is rejected with VX_E_BUSY, and so is a frame
  *     of a stream that already has max_queued_frames frames pending.
+ *   * Latency target (latency_target_ms > 0): the server keeps a running
+ *     estimate of its throughput (rays per ms of launch time).  A request is
+ *     rejected with VX_E_BUSY when the estimated time to finish the launch in
+ *     flight, the queued requests and itself exceeds the target, and each
+ *     launch takes queued requests in arrival order only up to half the
+ *     target's worth of rays (at least one request), so one launch never
+ *     holds the next for long.
  *   * Edits: single-voxel edits rebuild their chunk on host worker threads
  *     (generator fill + edits + vx_build_chunk); at the next launch boundary
  *     each rebuilt chunk is written to a fresh node-arena range (best-fit),
@@ -440,7 +447,7 @@ typedef struct vx_server_config {
     uint32_t max_views_per_launch;   /* default 64 */
     uint32_t edit_threads;           /* host threads rebuilding edited chunks (default 8) */
     uint32_t max_queued_frames;      /* per stream, queued + in flight; more is rejected (0: no limit) */
-    uint32_t pad;
+    float    latency_target_ms;      /* submit -> done target for admission and launch size (0: none) */
     vx_world_params world;           /* generator of the resident world */
     int32_t  grid_origin[3];
     uint32_t grid_dims[3];

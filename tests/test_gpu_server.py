@@ -121,6 +121,31 @@ def test_admission_budget(gpu):
     srv.close()
 
 
+def test_latency_target_admission(gpu):
+    """With a latency target, admission rejects a request once the server's
+    measured throughput says the queue and the request cannot finish within
+    it; before the first launch there is no estimate and nothing is rejected
+    for latency.  The results of admitted requests are unaffected."""
+    from oracle import oracle as O
+    from voxhip import scenes
+
+    sc, hw = host_world("c0")
+    clip, heights = scenes.scene_clipmap(sc)
+    ctx = _ctx(gpu, hw, clip, heights)
+    qcam = _scene_cam(gpu, sc, hw)
+    rays = scenes.query_rays(np.random.default_rng(8), 20000, sc.grid_origin, sc.grid_dims)
+    srv = gpu.Server(ctx, hw, memory_budget=1 << 30, latency_target_ms=0.01)
+    out = np.zeros(len(rays), gpu.HIT_DTYPE)
+    assert srv.submit_rays(qcam, rays, tag=1, out=out)  # no throughput estimate yet
+    _wait_all(srv, 1)
+    assert not srv.submit_rays(qcam, rays, tag=2)  # 20000 rays cannot finish within 0.01 ms
+    st = srv.stats()
+    assert st["rejected"] == 1 and st["completed"] == 1
+    ref = O.query_rays(O.World.from_host(hw, clip, heights), qcam, rays, gpu.HIT_DTYPE)
+    assert np.array_equal(out.view(np.uint8), ref.view(np.uint8))
+    srv.close()
+
+
 def test_edits_match_rebuilt_world(gpu):
     from voxhip import scenes
 

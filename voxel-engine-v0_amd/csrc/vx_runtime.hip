@@ -118,6 +118,9 @@ int dalloc(T** p, size_t count) {
   hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
   if (e != hipSuccess) return vx_fail(VX_E_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
   e = hipMemset(*p, 0, count * sizeof(T));
+  // The fill runs on the null stream, which the library's non-blocking
+  // streams do not wait for: finish it before any of them touches the memory.
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) return vx_fail(VX_E_DEVICE, "hipMemset: %s", hipGetErrorString(e));
   return VX_OK;
 }
@@ -644,7 +647,10 @@ int vx_view_counters(vx_view* v, uint64_t out[9], int reset) {
   unsigned long long tmp[16];
   VX_HIP(hipMemcpy(tmp, v->counters, sizeof(tmp), hipMemcpyDeviceToHost));
   for (int i = 0; i < 9; ++i) out[i] = tmp[i];
-  if (reset) VX_HIP(hipMemset(v->counters, 0, sizeof(tmp)));
+  if (reset) {
+    VX_HIP(hipMemset(v->counters, 0, sizeof(tmp)));
+    VX_HIP(hipStreamSynchronize(nullptr));  // before the view's non-blocking stream counts again
+  }
   return VX_OK;
 }
 

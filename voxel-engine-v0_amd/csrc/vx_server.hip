@@ -158,6 +158,12 @@ struct vx_server {
   std::deque<vx_result> outbox;
   std::unordered_map<uint32_t, Stream> streams;
   uint64_t mem_in_use = 0;
+  // Latency target: throughput estimate (rays per ms of launch time, 0 until
+  // the first launch), rays admitted and not yet launched, and the launch in
+  // flight (its rays and start time).
+  double rate = 0.0;
+  uint64_t queued_rays = 0, inflight_rays = 0;
+  double inflight_t0 = 0.0;
   bool stop = false;
   std::thread device;
   // stats
@@ -306,6 +312,10 @@ double pct(std::vector<double> v, double p) {
   return v[i];
 }
 
+uint64_t request_rays(const Pending& p) {
+  return p.req.kind == VX_REQ_VIEW ? (uint64_t)p.req.width * p.req.height : (uint64_t)p.rays.size();
+}
+
 void device_main(vx_server* s) {
   hipSetDevice(vx_internal_device(s->ctx));
   std::vector<vx_ray> all_rays;
@@ -321,26 +331,39 @@ void device_main(vx_server* s) {
       s->cv.wait(lk, [&] { return s->stop || !s->inbox.empty() || !s->ready.empty(); });
       if (s->stop) break;
       builds.swap(s->ready);
-      // Everything queued goes into this launch, at most one frame per view
-      // stream and max_views_per_launch views; the rest waits for the next.
+      // Queued requests join this launch in arrival order: at most one frame
+      // per view stream, max_views_per_launch views and, with a latency
+      // target, half the target's worth of rays (at least one request); the
+      // rest waits for the next launch.
+      const double cap = s->cfg.latency_target_ms > 0.0f && s->rate > 0.0 ? s->rate * 0.5 * s->cfg.latency_target_ms : 0.0;
+      uint64_t launch_rays = 0;
       std::deque<Pending> keep;
       std::vector<uint32_t> taken;
       while (!s->inbox.empty()) {
         Pending p = std::move(s->inbox.front());
         s->inbox.pop_front();
+        const uint64_t n = request_rays(p);
+        const bool full = cap > 0.0 && launch_rays > 0 && (double)(launch_rays + n) > cap;
         if (p.req.kind == VX_REQ_VIEW) {
           const bool dup = std::find(taken.begin(), taken.end(), p.req.stream_id) != taken.end();
-          if (dup || views.size() >= s->cfg.max_views_per_launch) {
+          if (full || dup || views.size() >= s->cfg.max_views_per_launch) {
             keep.push_back(std::move(p));
           } else {
             taken.push_back(p.req.stream_id);
+            launch_rays += n;
             views.push_back(std::move(p));
           }
+        } else if (full) {
+          keep.push_back(std::move(p));
         } else {
+          launch_rays += n;
           rays.push_back(std::move(p));
         }
       }
       s->inbox.swap(keep);
+      s->queued_rays -= launch_rays;
+      s->inflight_rays = launch_rays;
+      s->inflight_t0 = now_ms();
     }
     int rc = builds.empty() ? VX_OK : apply_ready(s, builds);
     const double t0 = now_ms();
@@ -409,6 +432,11 @@ void device_main(vx_server* s) {
       if (rc) s->last_error = rc;
       ++s->st.launches;
       s->st.busy_ms += t1 - t0;
+      if (!rc && s->inflight_rays && t1 > t0) {  // throughput estimate for the latency target
+        const double r = (double)s->inflight_rays / (t1 - t0);
+        s->rate = s->rate > 0.0 ? 0.8 * s->rate + 0.2 * r : r;
+      }
+      s->inflight_rays = 0;
       auto finish = [&](Pending& p, std::vector<double>& lat) {
         vx_result r{};
         r.tag = p.req.tag;
@@ -532,6 +560,19 @@ int vx_server_submit(vx_server* s, const vx_request* req) {
       return vx_fail(VX_E_BUSY, "vx_server_submit: %llu bytes over the %llu-byte budget",
                      (unsigned long long)(s->mem_in_use + need), (unsigned long long)s->cfg.memory_budget);
     }
+    const uint64_t n = request_rays(p);
+    if (s->cfg.latency_target_ms > 0.0f && s->rate > 0.0) {
+      // estimated finish: the rest of the launch in flight, then the queue, then this request
+      const double left = s->inflight_rays ? std::max(0.0, (double)s->inflight_rays / s->rate - (p.submit_ms - s->inflight_t0)) : 0.0;
+      const double est = left + (double)(s->queued_rays + n) / s->rate;
+      if (est > s->cfg.latency_target_ms) {
+        ++s->st.rejected;
+        if (req->kind == VX_REQ_VIEW && !s->streams[req->stream_id].w) s->streams.erase(req->stream_id);
+        return vx_fail(VX_E_BUSY, "vx_server_submit: estimated %.1f ms over the %.1f ms latency target", est,
+                       (double)s->cfg.latency_target_ms);
+      }
+    }
+    s->queued_rays += n;
     s->mem_in_use += need;
     if (req->kind == VX_REQ_VIEW) {
       Stream& stp = s->streams[req->stream_id];

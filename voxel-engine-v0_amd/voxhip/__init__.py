@@ -12,8 +12,10 @@ fails loudly when the library has not been built.
 from __future__ import annotations
 
 import ctypes as C
+import atexit
 import math
 import os
+import weakref
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -126,7 +128,7 @@ class QueryConfig(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_uint64), ("max_views_per_launch", C.c_uint32), ("edit_threads", C.c_uint32),
-                ("max_queued_frames", C.c_uint32), ("pad", C.c_uint32), ("world", WorldParams),
+                ("max_queued_frames", C.c_uint32), ("latency_target_ms", C.c_float), ("world", WorldParams),
                 ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_uint32 * 3)]
 
 
@@ -601,17 +603,32 @@ def query_tune(ctx: "Context", scene_cam: Camera, rays: DeviceArray, hits: Devic
     return cfg
 
 
+_LIVE_SERVERS: "weakref.WeakSet[Server]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_servers() -> None:
+    # A server's device thread uses its context: stop every server before
+    # interpreter teardown can destroy contexts in arbitrary order.
+    for srv in list(_LIVE_SERVERS):
+        try:
+            srv.close()
+        except Exception:
+            pass
+
+
 class Server:
     """The serving loop (vx_server_*): continuous batching of view-stream frames
     and ray batches over one context, with admission and an edit stream."""
 
     def __init__(self, ctx: "Context", world: "HostWorld", memory_budget: int, max_views_per_launch: int = 64,
-                 edit_threads: int = 8, max_queued_frames: int = 0):
+                 edit_threads: int = 8, max_queued_frames: int = 0, latency_target_ms: float = 0.0):
         cfg = ServerConfig()
         cfg.memory_budget = memory_budget
         cfg.max_views_per_launch = max_views_per_launch
         cfg.edit_threads = edit_threads
         cfg.max_queued_frames = max_queued_frames
+        cfg.latency_target_ms = latency_target_ms
         cfg.world = world.params
         for i in range(3):
             cfg.grid_origin[i] = world.grid_origin[i]
@@ -621,6 +638,7 @@ class Server:
         h = C.c_void_p()
         _check(LIB.vx_server_create(ctx._h, C.byref(view), C.byref(cfg), C.byref(h)), "vx_server_create")
         self._h = h
+        _LIVE_SERVERS.add(self)
         self.ctx, self.world = ctx, world
         self._keep = {}  # tag -> host buffers the server writes into
 
