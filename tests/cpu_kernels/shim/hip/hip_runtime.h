@@ -18,6 +18,9 @@ inline uint4 make_uint4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return
 inline uint32_t __float_as_uint(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 inline float __uint_as_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 inline int __clz(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+// The device's approximate square root (v_sqrt_f32); the pass code uses it only for
+// conservative culling bounds, so the correctly rounded one stands in here.
+#define __builtin_amdgcn_sqrtf(x) sqrtf(x)
 inline int __popc(uint32_t x) { return __builtin_popcount(x); }
 inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
 inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
