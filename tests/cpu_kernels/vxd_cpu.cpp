@@ -271,6 +271,15 @@ int vxd_cpu_convert(const float* x, uint32_t n, int32_t* out_i, uint32_t* out_u)
   return 0;
 }
 
+// The device value noise and fbm, elementwise: out[2n] = value_noise, fbm at (x, y).
+int vxd_cpu_noise(const float* x, const float* y, uint32_t n, float* out) {
+  for (uint32_t k = 0; k < n; ++k) {
+    out[2 * k + 0] = plain::value_noise(F2(x[k], y[k]));
+    out[2 * k + 1] = plain::fbm(F2(x[k], y[k]));
+  }
+  return 0;
+}
+
 // The device transcendentals, elementwise: out[5n] = sin, cos, exp, log, pow(x, y).
 int vxd_cpu_math(const float* x, const float* y, uint32_t n, float* out) {
   for (uint32_t k = 0; k < n; ++k) {

@@ -126,3 +126,23 @@ def test_device_transcendentals_match_oracle():
     bad = np.argwhere(~same)
     assert bad.size == 0, [(float(x[i]), float(y[i]), ["sin", "cos", "exp", "log", "pow"][j], float(got[i, j]),
                             float(want[i, j])) for i, j in bad[:8]]
+
+
+def test_device_value_noise_matches_oracle():
+    """The device value noise (integer corner hashes below 2^23, the per-corner
+    conversions above) and fbm against the oracle's, bit for bit, including
+    negative, lattice, huge and non-finite coordinates."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(12)
+    pts = [rng.uniform(-50.0, 50.0, (4000, 2)), rng.uniform(-1e6, 1e6, (2000, 2)),
+           rng.integers(-9000, 9000, (1000, 2)).astype(np.float64),
+           np.array([[8388607.0, 0.5], [8388608.0, -3.25], [-8388608.0, 1.0], [-8388609.5, 2.0], [1.6e7, 1.6e7],
+                     [3e9, -3e9], [-2.2e9, 7.0], [1e30, 1.0], [np.inf, 0.0], [np.nan, 1.0], [-0.0, -0.0],
+                     [0.99999994, -0.99999994], [-1e-40, 5.0]])]
+    xy = np.concatenate(pts).astype(np.float32)
+    got = vxd_cpu.noise(xy[:, 0], xy[:, 1])
+    L = O.LIB
+    want = np.array([[L.vxo_value_noise(float(a), float(b)), L.vxo_fbm(float(a), float(b))] for a, b in xy], np.float32)
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), xy[np.argwhere(~same)[:5, 0]].tolist()

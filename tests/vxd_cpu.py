@@ -51,6 +51,8 @@ def lib():
         _LIB.vxd_cpu_convert.restype = C.c_int
         _LIB.vxd_cpu_math.argtypes = [vp, vp, C.c_uint32, vp]
         _LIB.vxd_cpu_math.restype = C.c_int
+        _LIB.vxd_cpu_noise.argtypes = [vp, vp, C.c_uint32, vp]
+        _LIB.vxd_cpu_noise.restype = C.c_int
     return _LIB
 
 
@@ -95,4 +97,13 @@ def math(x: np.ndarray, y: np.ndarray) -> np.ndarray:
     y = np.ascontiguousarray(y, np.float32)
     out = np.zeros((len(x), 5), np.float32)
     assert lib().vxd_cpu_math(x.ctypes.data, y.ctypes.data, len(x), out.ctypes.data) == 0
+    return out
+
+
+def noise(x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """The device code's value_noise and fbm at (x, y), as an (n, 2) array."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    out = np.zeros((len(x), 2), np.float32)
+    assert lib().vxd_cpu_noise(x.ctypes.data, y.ctypes.data, len(x), out.ctypes.data) == 0
     return out
