@@ -4,7 +4,9 @@ Workload (one "step" = one frame): the 16x4x16-chunk seeded world with a
 5-level clipmap, frame f of the 120-frame seeded camera flight, rendered at
 3840x2160 (render scale 1.0) through primary -> local TAA -> godray ->
 composite -> composite TAA.  Inputs are resident in HBM before timing starts;
-each step writes the 464-byte camera and launches the frame.
+each step updates the clipmap for the frame's camera (Clipmap::update with the
+new texels generated on the device, as the reference's frame begins), writes
+the 464-byte camera and launches the frame.
 
 Prints one JSON line.  value = primary rays per second of the whole frame
 (rays per frame / wall time per frame).  Also reports the primary kernel's
@@ -51,6 +53,8 @@ def parse():
                     help="duration of the serving workload's arrival schedule (0: skip it)")
     ap.add_argument("--single-batch", action="store_true",
                     help="render the 4K view through a one-view batch (work-stealing launches) instead of vx_render")
+    ap.add_argument("--static-clipmap", action="store_true",
+                    help="keep frame 0's clipmap instead of updating it every frame")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline)")
     ap.add_argument("--cpu-rows", type=int, default=0, help="8-pixel tile rows of the CPU sample (0: the whole frame)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -133,6 +137,16 @@ def main():
     ctx.write_clipmap(clip, heights)
     view = vx.View(ctx, (W, H), (W, H))
     setup_s = time.time() - t0
+    # Every frame starts with the clipmap update (the reference's frame order:
+    # Clipmap::update, then its uploads, then the passes), heights generated on
+    # the device; frame 0's first touch rebuilds every level.
+    clip_state = None if a.static_clipmap else vx.ClipmapState(levels=5, res=256)
+
+    def clip_update(f):
+        if clip_state is not None:
+            pose, t_s = sc.poses[f % len(sc.poses)]
+            c, rects = clip_state.update(float(pose.pos[0]), float(pose.pos[2]), t_s)
+            ctx.write_clipmap_rects(c, sc.params, rects)
 
     cams = []
     prev = None
@@ -146,6 +160,7 @@ def main():
         one = vx.Batch(ctx, [view])
         render = one.render
     for f in range(a.warmup):
+        clip_update(f)
         view.write_camera(cams[f])
         render(vx.PASS_FRAME)
     ctx.sync()
@@ -157,6 +172,7 @@ def main():
     for k in range(a.steps):
         f = a.warmup + k
         t_f = time.perf_counter()
+        clip_update(f)
         view.write_camera(cams[f])
         render(vx.PASS_FRAME)
         if a.single_batch:
@@ -196,7 +212,9 @@ def main():
                     "kernel": "k_primary_batch", "algorithmic_bytes_per_launch": int(bytes_per_launch),
                     "algorithmic_bytes_per_ray": round(bytes_per_launch / rays, 1), "profiled": profiled_valu()}
 
-    lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup)
+    lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup,
+                                                    not a.static_clipmap)
+    clipmap = None if a.static_clipmap else clipmap_costs(vx, sc, ctx)
     batched = None if a.batch_views <= 0 else batching(vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps)
     queries = None if a.query_batches <= 0 else ray_queries(vx, scenes, a.query_batches)
     serving = None if a.serve_seconds <= 0 else serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms)
@@ -222,7 +240,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seed-42 world, seeded camera flight)",
         "config": {"workload": f"plain one-view-per-launch full pipeline at {W}x{H}, 16x4x16-chunk world, "
-                               f"5-level clipmap, camera flight frames {a.warmup}..{a.warmup + a.steps - 1}",
+                               f"5-level clipmap ({'static' if a.static_clipmap else 'updated every frame on the device'}), "
+                               f"camera flight frames {a.warmup}..{a.warmup + a.steps - 1}",
                    "resolution": [W, H], "chunks": hw.chunk_count, "nodes": int(len(hw.nodes)),
                    "path": "plain", "primary_kernels": "staged" if a.staged else "persistent work-stealing over the view's 8x8 tiles"},
         "primary_kernel_rays_per_s": round(rays / (prim_ms * 1e-3), 1),
@@ -232,6 +251,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "latency_1080p_ms": lat1080,
+        "clipmap_update": clipmap,
         "batched": batched,
         "ray_queries": queries,
         "serving": serving,
@@ -245,16 +265,21 @@ def main():
         copy_calibration(a.copy_calib)
 
 
-def latency_1080p(vx, scenes, sc, hw, ctx, steps, warmup):
+def latency_1080p(vx, scenes, sc, hw, ctx, steps, warmup, clip_every_frame=True):
     """Per-view frame latency of one 1920x1080 view along the same flight
-    (camera write -> all five passes done), p50/p99 in ms."""
+    (clipmap update, camera write -> all five passes done), p50/p99 in ms."""
     res = (1920, 1080)
     view = vx.View(ctx, res, res)
+    st = vx.ClipmapState(levels=5, res=256) if clip_every_frame else None
     lat, prev = [], None
     for f in range(warmup + steps):
         cam = scenes.frame_camera(sc, hw, f, prev_view_proj=prev, jitter=True, render=res)
         prev = np.array(cam.view_proj, np.float32)
         t = time.perf_counter()
+        if st is not None:
+            pose, t_s = sc.poses[f % len(sc.poses)]
+            c, rects = st.update(float(pose.pos[0]), float(pose.pos[2]), t_s)
+            ctx.write_clipmap_rects(c, sc.params, rects)
         view.write_camera(cam)
         view.render(vx.PASS_FRAME)
         view.timings()
@@ -263,6 +288,43 @@ def latency_1080p(vx, scenes, sc, hw, ctx, steps, warmup):
     view.close()
     return {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
             "frames": steps}
+
+
+def clipmap_costs(vx, sc, ctx, rebuilds=5):
+    """Clipmap::update and its uploads per frame along the 120-frame flight, and
+    a whole-clipmap rebuild (first touch: 5 levels x 256^2 texels), two ways:
+    heights on the host then patch uploads (the reference's path), and heights
+    generated on the device by the fill kernel.  Wall milliseconds per call."""
+    def run(device, frames):
+        st = vx.ClipmapState(levels=5, res=256)
+        per, texels = [], 0
+        for x, z, t in frames:
+            t0 = time.perf_counter()
+            c, rects = st.update(x, z, t)
+            if device:
+                ctx.write_clipmap_rects(c, sc.params, rects)
+            else:
+                ctx.write_clipmap_patches(c, rects, vx.clipmap_fill_rects(sc.params, rects))
+            per.append((time.perf_counter() - t0) * 1e3)
+            texels += sum(r.w * r.h for r in rects)
+        return per, texels
+
+    flight = [(float(p.pos[0]), float(p.pos[2]), t) for p, t in sc.poses]
+    out = {}
+    for name, device in (("host_fill", False), ("device_fill", True)):
+        full = []
+        for _ in range(rebuilds):
+            per, _ = run(device, flight[:1])
+            full.append(per[0])
+        per, texels = run(device, flight)
+        steady = per[1:]
+        out[name] = {"rebuild_ms": round(float(np.median(full)), 3),
+                     "frame_ms_mean": round(float(np.mean(steady)), 4),
+                     "frame_ms_p99": round(float(np.percentile(steady, 99)), 4),
+                     "texels_per_frame": round((texels - 5 * 256 * 256) / max(len(steady), 1), 1)}
+    out["rebuild_texels"] = 5 * 256 * 256
+    out["rebuild_speedup"] = round(out["host_fill"]["rebuild_ms"] / max(out["device_fill"]["rebuild_ms"], 1e-9), 2)
+    return out
 
 
 def batching(vx, scenes, sc, hw, ctx, n_views, steps):

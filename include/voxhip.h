@@ -196,6 +196,19 @@ typedef struct vx_clip_patch {
     const float* data;     /* w*h floats, row-major */
 } vx_clip_patch;
 int vx_write_clipmap(vx_ctx* ctx, const vx_clipmap* params, const vx_clip_patch* patches, uint32_t n);
+/* Renderer::write_clipmap_updates with the heights generated on the device:
+ * one kernel computes every texel of the rectangles (the same heights as
+ * vx_clipmap_fill_rects, bit for bit) straight into the resident clip array,
+ * then the uniform is written; stream-ordered before the next vx_render, like
+ * vx_write_clipmap.  At most VX_CLIP_RECTS_MAX rectangles; `world` is the seeded
+ * world generator's parameters (section 2). */
+struct vx_world_params;
+struct vx_clip_rect;
+int vx_write_clipmap_rects(vx_ctx* ctx, const vx_clipmap* params, const struct vx_world_params* world,
+                           const struct vx_clip_rect* rects, uint32_t n);
+/* Copies the resident clip heights (clip_levels * clip_res^2 floats, storage
+ * order) to host memory; bytes must cover them. */
+int vx_read_clipmap(vx_ctx* ctx, float* heights, size_t bytes);
 
 /* A view: one camera stream with its own history (the reference's texture set,
  * render/state/textures.rs:122-285, and ping-pong, bindgroups.rs:305-459).
@@ -408,6 +421,64 @@ int vx_camera_build(const vx_pose* pose, uint32_t render_w, uint32_t render_h,
  * heights: levels*res*res floats, layer-major, row-major (storage coordinates). */
 int vx_clipmap_build(const vx_world_params* p, float cam_x, float cam_z, uint32_t levels,
                      uint32_t res, float base_cell_m, vx_clipmap* out, float* heights);
+
+/* Incremental clipmap: the state of Clipmap (clipmap.rs:44-60) and its per-frame
+ * Clipmap::update (clipmap.rs:182-577).  Per level the state keeps the committed
+ * origin cell and toroidal offset (storage = (logical + offset) mod res).  An
+ * update snaps every level's origin to the camera, then, if the frame updates
+ * (a level is new, the camera is within VX_CLIP_EDGE_GUARD texels of a level's
+ * committed edge, or a level moved and min_update_interval_s has passed since
+ * the last update, clipmap.rs:276-284), scrolls each moved level's offset by its
+ * origin's cell delta and plans the texels that entered it as rectangles, in
+ * the reference's upload order: the whole level on first touch or on a jump of
+ * res cells or more, else the new columns then the new rows, each split in two
+ * where it wraps.  It writes the committed uniform to *params (levels past the
+ * state's levels: origin 0, cell 1, offset 0).
+ *
+ * A row strip spans the whole width and a column strip the whole height.  The
+ * reference writes that spanning axis from storage 0 in logical order
+ * (clipmap.rs:376-377,470-471), which agrees with the sampler's
+ * storage = (logical + offset) mod res (clipmap.wgsl:42-47) only while that
+ * axis' offset is 0: once both axes have scrolled, its new strips hold their
+ * heights rotated along their length.  That placement is the default (the
+ * drop-in reproduces the reference's texture); VX_CLIP_WRAP_FIX places the
+ * spanning axis at its wrapped storage position, so that every update leaves the
+ * texture equal to a full rebuild at the committed origins. */
+typedef struct vx_clipmap_state vx_clipmap_state;
+
+/* Texels of one level that an update rewrites: the storage rectangle
+ * (x, y, w, h), row-major, where storage texel (x+i, y+j) holds logical texel
+ * ((lx0+i) mod res, (lz0+j) mod res) of a level with origin
+ * (origin_x_m, origin_z_m) and cell cell_m: the height of the voxel column under
+ * that texel's centre (clipmap.rs:93-170).  48 bytes. */
+typedef struct vx_clip_rect {
+    uint32_t level;
+    uint32_t x, y, w, h;
+    uint32_t lx0, lz0;
+    uint32_t res;
+    float    origin_x_m, origin_z_m, cell_m;
+    uint32_t pad;
+} vx_clip_rect;
+
+#define VX_CLIP_WRAP_FIX     1u
+#define VX_CLIP_EDGE_GUARD   16    /* clipmap.rs:207 */
+#define VX_CLIP_RECTS_MAX    64u   /* 4 per level: columns and rows, each split once */
+
+/* Clipmap::new (clipmap.rs:52-58): every level untouched. */
+int  vx_clipmap_state_create(uint32_t levels, uint32_t res, float base_cell_m, float min_update_interval_s,
+                             uint32_t flags, vx_clipmap_state** out);
+void vx_clipmap_state_destroy(vx_clipmap_state* s);
+/* Clipmap::invalidate_all (clipmap.rs:580-585): the next update rebuilds every level. */
+int  vx_clipmap_invalidate(vx_clipmap_state* s);
+/* Clipmap::update (clipmap.rs:182-577) without the heights: rects needs room for
+ * 4 * levels entries (VX_E_INVALID otherwise, before any state changes). */
+int  vx_clipmap_update(vx_clipmap_state* s, float cam_x, float cam_z, float time_s, vx_clipmap* params,
+                       vx_clip_rect* rects, uint32_t cap, uint32_t* n_rects);
+/* The heights of rectangles on the host (the reference's build_full_level,
+ * build_col_patch and build_row_patch, clipmap.rs:107-170), rectangle after
+ * rectangle, w*h floats each, row-major: ClipmapUpload::data_f32.  With
+ * storage-coordinate rectangles these are vx_write_clipmap's patches. */
+int  vx_clipmap_fill_rects(const vx_world_params* p, const vx_clip_rect* rects, uint32_t n, float* data);
 
 /* ------------------------------------------------------------------------ */
 /* 3. Serving                                                               */

@@ -13,6 +13,7 @@
 //     clipmap fill (clipmap.rs:62-122).
 #include "voxhip.h"
 #include "vx_internal.h"
+#include "vx_terrain.h"
 
 #include <algorithm>
 #include <atomic>
@@ -300,31 +301,8 @@ void ropes_rec(const vx_node* nodes, vx_node_ropes* ropes, uint32_t idx) {
 // ---------------------------------------------------------------------------
 // Synthetic world.
 // ---------------------------------------------------------------------------
-inline uint32_t mix32(uint32_t h) {
-  // Murmur3 32-bit finaliser.
-  h ^= h >> 16;
-  h *= 0x85ebca6bu;
-  h ^= h >> 13;
-  h *= 0xc2b2ae35u;
-  h ^= h >> 16;
-  return h;
-}
-inline uint32_t lattice_hash(int32_t x, int32_t z, uint32_t salt) {
-  return mix32((uint32_t)x * 0x9E3779B1u ^ mix32((uint32_t)z * 0x85EBCA77u ^ salt));
-}
-inline double lattice_value(int32_t x, int32_t z, uint32_t salt) {
-  return (double)(lattice_hash(x, z, salt) >> 8) * (1.0 / 16777216.0);
-}
-double value_noise_2d(double x, double z, uint32_t salt) {
-  double fx = std::floor(x), fz = std::floor(z);
-  int32_t ix = (int32_t)fx, iz = (int32_t)fz;
-  double tx = x - fx, tz = z - fz;
-  double ux = tx * tx * (3.0 - 2.0 * tx), uz = tz * tz * (3.0 - 2.0 * tz);
-  double a = lattice_value(ix, iz, salt), b = lattice_value(ix + 1, iz, salt);
-  double c = lattice_value(ix, iz + 1, salt), d = lattice_value(ix + 1, iz + 1, salt);
-  double x0 = a + (b - a) * ux, x1 = c + (d - c) * ux;
-  return x0 + (x1 - x0) * uz;
-}
+using vxt::lattice_hash;
+using vxt::mix32;
 
 struct TreeInfo {
   bool present;
@@ -418,11 +396,7 @@ void vx_world_default(vx_world_params* p) {
 }
 
 int32_t vx_world_ground_height(const vx_world_params* p, int32_t wx, int32_t wz) {
-  double l0 = (double)(p->wavelength0 > 0 ? p->wavelength0 : 64);
-  double o0 = value_noise_2d((double)wx / l0, (double)wz / l0, p->seed * 0x9E3779B9u + 0x1234u);
-  double o1 = value_noise_2d((double)wx / (0.5 * l0), (double)wz / (0.5 * l0), p->seed * 0x9E3779B9u + 0xBEEFu);
-  double n = (o0 + 0.5 * o1) / 1.5;  // [0,1]
-  return p->base_y + (int32_t)std::floor((double)p->amplitude * (2.0 * n - 1.0) + 0.5);
+  return vxt::ground_height(*p, wx, wz);
 }
 
 int vx_world_fill_chunk(const vx_world_params* p, int32_t cx, int32_t cy, int32_t cz,
@@ -840,12 +814,11 @@ int vx_clipmap_build(const vx_world_params* p, float cam_x, float cam_z, uint32_
   out->levels = levels;
   out->res = res;
   out->base_cell_m = base_cell_m;
-  const float vs = 0.1f;
   for (uint32_t l = 0; l < VX_CLIP_LEVELS_MAX; ++l) {
     const float cell = base_cell_m * (float)(1u << l);
     const float half = (float)res * 0.5f;
-    const int32_t ocx = (int32_t)std::floor((cam_x - half * cell) / cell);
-    const int32_t ocz = (int32_t)std::floor((cam_z - half * cell) / cell);
+    const int32_t ocx = vxt::sat_i32(std::floor((cam_x - half * cell) / cell));
+    const int32_t ocz = vxt::sat_i32(std::floor((cam_z - half * cell) / cell));
     const float oxm = (float)ocx * cell, ozm = (float)ocz * cell;
     int32_t offx = ocx % (int32_t)res, offz = ocz % (int32_t)res;
     if (offx < 0) offx += (int32_t)res;
@@ -859,15 +832,216 @@ int vx_clipmap_build(const vx_world_params* p, float cam_x, float cam_z, uint32_
     if (l >= levels) continue;
     float* layer = heights + (size_t)l * res * res;
     for (uint32_t tz = 0; tz < res; ++tz) {
-      const float wz = ozm + ((float)tz + 0.5f) * cell;
-      const int32_t vz = (int32_t)std::floor(wz / vs);
       const uint32_t sz = (tz + (uint32_t)offz) % res;
-      for (uint32_t tx = 0; tx < res; ++tx) {
-        const float wx = oxm + ((float)tx + 0.5f) * cell;
-        const int32_t vx = (int32_t)std::floor(wx / vs);
-        const uint32_t sx = (tx + (uint32_t)offx) % res;
-        layer[(size_t)sz * res + sx] = (float)vx_world_ground_height(p, vx, vz) * vs;
+      for (uint32_t tx = 0; tx < res; ++tx)
+        layer[(size_t)sz * res + (tx + (uint32_t)offx) % res] = vxt::clip_texel_height(*p, oxm, ozm, cell, tx, tz);
+    }
+  }
+  return VX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Incremental clipmap (clipmap.rs:44-60 state, 182-577 update, 580-585
+// invalidate).  Per level the state holds the committed origin cell and the
+// toroidal offset (storage = (logical + offset) mod res); an update scrolls the
+// offset by the origin's cell delta and plans the texels that entered the
+// level as storage rectangles, in the reference's upload order: a full level on
+// first touch or a jump of res cells or more, else the new columns (split in
+// two where they wrap) then the new rows.
+// ---------------------------------------------------------------------------
+struct vx_clipmap_state {
+  uint32_t levels = 0, res = 0, flags = 0;
+  float base_cell = 1.0f, min_interval = 0.0f;
+  int32_t cell[VX_CLIP_LEVELS_MAX][2];
+  uint32_t off[VX_CLIP_LEVELS_MAX][2];
+  float last_time = -INFINITY;
+
+  void invalidate() {
+    for (uint32_t l = 0; l < VX_CLIP_LEVELS_MAX; ++l) {
+      cell[l][0] = cell[l][1] = INT32_MIN;
+      off[l][0] = off[l][1] = 0;
+    }
+    last_time = -INFINITY;
+  }
+};
+
+namespace {
+
+// Logical texel of the camera in a level whose origin cell is `o`.
+inline int32_t cam_texel(float cam, int32_t o, float cell) { return vxt::sat_i32(std::floor((cam - (float)o * cell) / cell)); }
+
+inline uint32_t wrap_mod(int64_t v, uint32_t m) {
+  int64_t r = v % (int64_t)m;
+  return (uint32_t)(r < 0 ? r + (int64_t)m : r);
+}
+
+struct LevelPlan {
+  float cell, ox, oz;  // new origin (metres) and cell size
+  int32_t cx, cz;      // new origin cell
+  int64_t dx, dz;      // cell delta from the committed origin (0 on first touch)
+  bool first, moved;
+};
+
+}  // namespace
+
+extern "C" {
+
+int vx_clipmap_state_create(uint32_t levels, uint32_t res, float base_cell_m, float min_update_interval_s,
+                            uint32_t flags, vx_clipmap_state** out) {
+  if (!out || levels == 0 || levels > VX_CLIP_LEVELS_MAX || res < 2 * VX_CLIP_EDGE_GUARD + 1 || res > 4096 ||
+      !(base_cell_m > 0.0f) || (flags & ~VX_CLIP_WRAP_FIX))
+    return vx_fail(VX_E_INVALID, "vx_clipmap_state_create: bad argument");
+  vx_clipmap_state* s = new (std::nothrow) vx_clipmap_state();
+  if (!s) return vx_fail(VX_E_NOMEM, "vx_clipmap_state_create: host allocation");
+  s->levels = levels;
+  s->res = res;
+  s->base_cell = base_cell_m;
+  s->min_interval = min_update_interval_s;
+  s->flags = flags;
+  s->invalidate();
+  *out = s;
+  return VX_OK;
+}
+
+void vx_clipmap_state_destroy(vx_clipmap_state* s) { delete s; }
+
+int vx_clipmap_invalidate(vx_clipmap_state* s) {
+  if (!s) return vx_fail(VX_E_INVALID, "vx_clipmap_invalidate: null state");
+  s->invalidate();
+  return VX_OK;
+}
+
+int vx_clipmap_update(vx_clipmap_state* s, float cam_x, float cam_z, float time_s, vx_clipmap* params,
+                      vx_clip_rect* rects, uint32_t cap, uint32_t* n_rects) {
+  if (!s || !params || !n_rects || !rects) return vx_fail(VX_E_INVALID, "vx_clipmap_update: null argument");
+  if (cap < 4 * s->levels)
+    return vx_fail(VX_E_INVALID, "vx_clipmap_update: room for %u rectangles, an update may emit %u", cap, 4 * s->levels);
+  const uint32_t res = s->res;
+  const int32_t guard = VX_CLIP_EDGE_GUARD, hi = (int32_t)res - VX_CLIP_EDGE_GUARD;
+  const float half = (float)res * 0.5f;
+
+  // Where every level wants to be, and whether any level is new, moved or
+  // has the camera within the guard band of its committed window.
+  LevelPlan plan[VX_CLIP_LEVELS_MAX];
+  bool any_first = false, any_moved = false, any_force = false;
+  for (uint32_t l = 0; l < s->levels; ++l) {
+    LevelPlan& q = plan[l];
+    q.cell = s->base_cell * (float)(1u << l);
+    q.cx = vxt::sat_i32(std::floor((cam_x - half * q.cell) / q.cell));
+    q.cz = vxt::sat_i32(std::floor((cam_z - half * q.cell) / q.cell));
+    q.ox = (float)q.cx * q.cell;
+    q.oz = (float)q.cz * q.cell;
+    q.first = s->cell[l][0] == INT32_MIN;
+    q.dx = q.first ? 0 : (int64_t)q.cx - s->cell[l][0];
+    q.dz = q.first ? 0 : (int64_t)q.cz - s->cell[l][1];
+    q.moved = q.first || q.dx != 0 || q.dz != 0;
+    bool force = q.first;
+    if (!q.first) {
+      const int32_t tx = cam_texel(cam_x, s->cell[l][0], q.cell), tz = cam_texel(cam_z, s->cell[l][1], q.cell);
+      force = tx < guard || tz < guard || tx >= hi || tz >= hi;
+    }
+    any_first |= q.first;
+    any_moved |= q.moved;
+    any_force |= force;
+  }
+  // One cadence gate for all levels.
+  const bool cadence_ok = any_first || (time_s - s->last_time) >= s->min_interval;
+  const bool update = any_first || any_force || (cadence_ok && any_moved);
+  if (update) s->last_time = time_s;
+
+  uint32_t n = 0;
+  auto emit = [&](uint32_t l, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t lx0, uint32_t lz0) {
+    vx_clip_rect& r = rects[n++];
+    r.level = l;
+    r.x = x;
+    r.y = y;
+    r.w = w;
+    r.h = h;
+    r.lx0 = lx0;
+    r.lz0 = lz0;
+    r.res = res;
+    r.origin_x_m = plan[l].ox;
+    r.origin_z_m = plan[l].oz;
+    r.cell_m = plan[l].cell;
+    r.pad = 0;
+  };
+  // A strip of `k` new logical lines starting at logical line `first` along
+  // one axis, at storage line (first + off) mod res, split where it wraps.
+  // `along` is the logical start of the strip's full-length axis at storage 0.
+  auto strip = [&](uint32_t l, bool cols, uint32_t first, uint32_t k, uint32_t off, uint32_t along) {
+    const uint32_t s0 = (first + off) % res;
+    const uint32_t a = s0 + k <= res ? k : res - s0;
+    if (cols) {
+      emit(l, s0, 0, a, res, first, along);
+      if (a < k) emit(l, 0, 0, k - a, res, first + a, along);
+    } else {
+      emit(l, 0, s0, res, a, along, first);
+      if (a < k) emit(l, 0, 0, res, k - a, along, first + a);
+    }
+  };
+
+  const bool fix = (s->flags & VX_CLIP_WRAP_FIX) != 0;
+  for (uint32_t l = 0; l < s->levels; ++l) {
+    const LevelPlan& q = plan[l];
+    if (update && q.moved) {
+      const uint64_t adx = (uint64_t)(q.dx < 0 ? -q.dx : q.dx), adz = (uint64_t)(q.dz < 0 ? -q.dz : q.dz);
+      if (q.first || adx >= res || adz >= res) {
+        s->off[l][0] = s->off[l][1] = 0;
+        emit(l, 0, 0, res, res, 0, 0);
+      } else {
+        s->off[l][0] = wrap_mod((int64_t)s->off[l][0] + q.dx, res);
+        s->off[l][1] = wrap_mod((int64_t)s->off[l][1] + q.dz, res);
+        // A strip spans the whole other axis.  The reference writes that axis
+        // from storage 0 in logical order (clipmap.rs:376-377,470-471), which
+        // agrees with the sampler's storage = (logical + offset) mod res only
+        // while that axis' offset is 0; VX_CLIP_WRAP_FIX rotates it by the offset.
+        const uint32_t along_z = fix ? (res - s->off[l][1]) % res : 0;
+        const uint32_t along_x = fix ? (res - s->off[l][0]) % res : 0;
+        if (q.dx != 0) strip(l, true, q.dx > 0 ? res - (uint32_t)adx : 0, (uint32_t)adx, s->off[l][0], along_z);
+        if (q.dz != 0) strip(l, false, q.dz > 0 ? res - (uint32_t)adz : 0, (uint32_t)adz, s->off[l][1], along_x);
       }
+      s->cell[l][0] = q.cx;
+      s->cell[l][1] = q.cz;
+    }
+  }
+
+  // The committed state: heights and uniform stay consistent (clipmap.rs:560-575).
+  std::memset(params, 0, sizeof(*params));
+  params->levels = s->levels;
+  params->res = res;
+  params->base_cell_m = s->base_cell;
+  for (uint32_t l = 0; l < VX_CLIP_LEVELS_MAX; ++l) {
+    if (l >= s->levels) {
+      params->level[l][2] = 1.0f;
+      continue;
+    }
+    const LevelPlan& q = plan[l];
+    const bool committed = s->cell[l][0] != INT32_MIN;
+    params->level[l][0] = committed ? (float)s->cell[l][0] * q.cell : q.ox;
+    params->level[l][1] = committed ? (float)s->cell[l][1] * q.cell : q.oz;
+    params->level[l][2] = q.cell;
+    params->offset[l][0] = s->off[l][0];
+    params->offset[l][1] = s->off[l][1];
+  }
+  *n_rects = n;
+  return VX_OK;
+}
+
+int vx_clipmap_fill_rects(const vx_world_params* p, const vx_clip_rect* rects, uint32_t n, float* data) {
+  if (!p || (n && (!rects || !data))) return vx_fail(VX_E_INVALID, "vx_clipmap_fill_rects: null argument");
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_clip_rect& r = rects[i];
+    if (!r.res || r.w > r.res || r.h > r.res || r.lx0 >= r.res || r.lz0 >= r.res)
+      return vx_fail(VX_E_INVALID, "vx_clipmap_fill_rects: rectangle %u out of range", i);
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_clip_rect& r = rects[i];
+    for (uint32_t j = 0; j < r.h; ++j) {
+      const uint32_t lz = (r.lz0 + j) % r.res;
+      for (uint32_t k = 0; k < r.w; ++k)
+        *data++ = vxt::clip_texel_height(*p, r.origin_x_m, r.origin_z_m, r.cell_m, (r.lx0 + k) % r.res, lz);
     }
   }
   return VX_OK;

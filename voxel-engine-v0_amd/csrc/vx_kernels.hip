@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "vx_launch.h"
+#include "vx_terrain.h"
 #include "vxd_consts.h"
 #include "vxd_scene.h"
 
@@ -591,6 +592,36 @@ __global__ __launch_bounds__(256) void k_chunk_lights(Scene S, uint32_t* __restr
 
 hipError_t vx_launch_chunk_lights(const vxd::Scene& S, uint32_t* out, uint32_t n_slots, hipStream_t st) {
   if (n_slots) hipLaunchKernelGGL(k_chunk_lights, dim3(n_slots), dim3(256), 0, st, S, out);
+  return hipGetLastError();
+}
+
+// Clipmap texels on the device: workgroup (bx, r) computes texels
+// [256 bx, 256 bx + 256) of rectangle r in row-major order and stores each at
+// its storage position; the value is the host producer's vxt::clip_texel_height.
+// Rectangles of one update may overlap (a column strip and a row strip share a
+// corner, and with the reference's strip placement they disagree there): as
+// with uploads applied in order, a texel is written only by the last rectangle
+// covering it.  Rectangles are checked against the array (level, extent, res)
+// on the host.
+__global__ __launch_bounds__(256) void k_clip_fill(const vx_clip_rect* __restrict__ rects, uint32_t n, vx_world_params wp,
+                                                   float* __restrict__ clip_height) {
+  const vx_clip_rect r = rects[blockIdx.y];
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= r.w * r.h) return;
+  const uint32_t j = t / r.w, i = t - j * r.w;
+  const uint32_t sx = r.x + i, sy = r.y + j;
+  for (uint32_t q = blockIdx.y + 1; q < n; ++q) {
+    const vx_clip_rect o = rects[q];
+    if (o.level == r.level && sx - o.x < o.w && sy - o.y < o.h) return;  // unsigned: also rejects sx < o.x
+  }
+  const float h = vxt::clip_texel_height(wp, r.origin_x_m, r.origin_z_m, r.cell_m, (r.lx0 + i) % r.res, (r.lz0 + j) % r.res);
+  clip_height[((size_t)r.level * r.res + sy) * r.res + sx] = h;
+}
+
+hipError_t vx_launch_clip_fill(const vx_clip_rect* rects, uint32_t n, uint32_t max_texels, const vx_world_params& world,
+                               float* clip_height, hipStream_t st) {
+  if (n && max_texels)
+    hipLaunchKernelGGL(k_clip_fill, dim3((max_texels + 255u) / 256u, n), dim3(256), 0, st, rects, n, world, clip_height);
   return hipGetLastError();
 }
 

@@ -30,3 +30,7 @@ hipError_t vx_launch_query(const vxd::Scene& S, const vx_camera* cam_dev, const 
                            uint32_t* cursor, uint32_t persistent, uint32_t refill_below, hipStream_t st);
 // Per chunk slot: 1 when its node range holds a LIGHT leaf, else 0 (n_slots entries).
 hipError_t vx_launch_chunk_lights(const vxd::Scene& S, uint32_t* out, uint32_t n_slots, hipStream_t st);
+// Clipmap texels of n rectangles (device array, at most max_texels texels each)
+// from the world generator, stored at their storage positions in clip_height.
+hipError_t vx_launch_clip_fill(const vx_clip_rect* rects, uint32_t n, uint32_t max_texels, const vx_world_params& world,
+                               float* clip_height, hipStream_t st);

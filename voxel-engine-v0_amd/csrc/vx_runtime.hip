@@ -54,6 +54,7 @@ struct vx_ctx {
   uint32_t* grid = nullptr;
   float* clip_height = nullptr;
   vx_clipmap* clip_dev = nullptr;
+  vx_clip_rect* clip_rects = nullptr;  // VX_CLIP_RECTS_MAX rectangles for the device clipmap fill
   // Per chunk slot: whether its nodes hold a LIGHT leaf (derived on the device
   // from the node arena and the chunk table after they change).
   uint32_t* chunk_light = nullptr;
@@ -261,7 +262,8 @@ int vx_create(const vx_caps* caps, vx_ctx** out) {
       (rc = dalloc(&ctx->meta, ctx->chunk_cap)) || (rc = dalloc(&ctx->macro, (size_t)ctx->chunk_cap * 16)) ||
       (rc = dalloc(&ctx->colinfo, (size_t)ctx->chunk_cap * 2048)) || (rc = dalloc(&ctx->grid, ctx->grid_cap)) ||
       (rc = dalloc(&ctx->clip_height, (size_t)ctx->clip_levels * ctx->clip_res * ctx->clip_res)) ||
-      (rc = dalloc(&ctx->clip_dev, 1)) || (rc = dalloc(&ctx->chunk_light, ctx->chunk_cap))) {
+      (rc = dalloc(&ctx->clip_dev, 1)) || (rc = dalloc(&ctx->clip_rects, VX_CLIP_RECTS_MAX)) ||
+      (rc = dalloc(&ctx->chunk_light, ctx->chunk_cap))) {
     vx_destroy(ctx);
     return rc;
   }
@@ -292,6 +294,7 @@ void vx_destroy(vx_ctx* ctx) {
   dfree(ctx->grid);
   dfree(ctx->clip_height);
   dfree(ctx->clip_dev);
+  dfree(ctx->clip_rects);
   dfree(ctx->chunk_light);
   dfree(ctx->qcam_dev);
   dfree(ctx->qcursor);
@@ -360,6 +363,42 @@ int vx_write_clipmap(vx_ctx* ctx, const vx_clipmap* params, const vx_clip_patch*
                             ctx->stream));
   }
   VX_HIP(hipMemcpyAsync(ctx->clip_dev, params, sizeof(vx_clipmap), hipMemcpyHostToDevice, ctx->stream));
+  VX_HIP(hipStreamSynchronize(ctx->stream));
+  return VX_OK;
+}
+
+int vx_write_clipmap_rects(vx_ctx* ctx, const vx_clipmap* params, const vx_world_params* world, const vx_clip_rect* rects,
+                           uint32_t n) {
+  if (!ctx || !params || !world || (!rects && n)) return vx_fail(VX_E_INVALID, "vx_write_clipmap_rects: null argument");
+  if (params->levels > ctx->clip_levels || (params->levels && params->res != ctx->clip_res))
+    return vx_fail(VX_E_CAPACITY, "vx_write_clipmap_rects: levels/res (%u, %u) do not fit (%u, %u)", params->levels,
+                   params->res, ctx->clip_levels, ctx->clip_res);
+  if (n > VX_CLIP_RECTS_MAX)
+    return vx_fail(VX_E_INVALID, "vx_write_clipmap_rects: %u rectangles (at most %u)", n, VX_CLIP_RECTS_MAX);
+  uint32_t max_texels = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_clip_rect& r = rects[i];
+    if (r.level >= ctx->clip_levels || r.res != ctx->clip_res || r.w > r.res || r.h > r.res || r.x >= r.res ||
+        r.y >= r.res || r.x + r.w > r.res || r.y + r.h > r.res || r.lx0 >= r.res || r.lz0 >= r.res)
+      return vx_fail(VX_E_INVALID, "vx_write_clipmap_rects: rectangle %u out of range", i);
+    max_texels = std::max(max_texels, r.w * r.h);
+  }
+  DeviceGuard g(ctx->device);
+  if (n && max_texels) {
+    VX_HIP(hipMemcpyAsync(ctx->clip_rects, rects, n * sizeof(vx_clip_rect), hipMemcpyHostToDevice, ctx->stream));
+    VX_HIP(vx_launch_clip_fill(ctx->clip_rects, n, max_texels, *world, ctx->clip_height, ctx->stream));
+  }
+  VX_HIP(hipMemcpyAsync(ctx->clip_dev, params, sizeof(vx_clipmap), hipMemcpyHostToDevice, ctx->stream));
+  VX_HIP(hipStreamSynchronize(ctx->stream));
+  return VX_OK;
+}
+
+int vx_read_clipmap(vx_ctx* ctx, float* heights, size_t bytes) {
+  if (!ctx || !heights) return vx_fail(VX_E_INVALID, "vx_read_clipmap: null argument");
+  const size_t need = (size_t)ctx->clip_levels * ctx->clip_res * ctx->clip_res * sizeof(float);
+  if (bytes < need) return vx_fail(VX_E_INVALID, "vx_read_clipmap: %zu bytes, need %zu", bytes, need);
+  DeviceGuard g(ctx->device);
+  VX_HIP(hipMemcpyAsync(heights, ctx->clip_height, need, hipMemcpyDeviceToHost, ctx->stream));
   VX_HIP(hipStreamSynchronize(ctx->stream));
   return VX_OK;
 }

@@ -48,6 +48,9 @@ ARENA_NODES, ARENA_ROPES, ARENA_META, ARENA_MACRO, ARENA_COLINFO = range(5)
 CTX_GODRAY_MACRO_FIX = 1
 CTX_PRIMARY_STAGED = 2
 CAM_JITTER = 1
+CLIP_WRAP_FIX = 1
+CLIP_EDGE_GUARD = 16
+CLIP_RECTS_MAX = 64
 
 NODE_DTYPE = np.dtype([("child_base", "<u4"), ("child_mask", "<u4"), ("material", "<u4"), ("key", "<u4")])
 ROPES_DTYPE = np.dtype([(n, "<u4") for n in ("px", "nx", "py", "ny", "pz", "nz", "pad0", "pad1")])
@@ -99,6 +102,20 @@ class Region(C.Structure):
 class ClipPatch(C.Structure):
     _fields_ = [("level", C.c_uint32), ("x", C.c_uint32), ("y", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
                 ("data", C.c_void_p)]
+
+
+class ClipRect(C.Structure):
+    """vx_clip_rect: storage texels (x, y, w, h) of one level holding logical
+    texels ((lx0 + i) mod res, (lz0 + j) mod res) of a level with the given
+    origin and cell (48 bytes)."""
+
+    _fields_ = [("level", C.c_uint32), ("x", C.c_uint32), ("y", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
+                ("lx0", C.c_uint32), ("lz0", C.c_uint32), ("res", C.c_uint32), ("origin_x_m", C.c_float),
+                ("origin_z_m", C.c_float), ("cell_m", C.c_float), ("pad", C.c_uint32)]
+
+    def as_tuple(self) -> tuple:
+        return (self.level, self.x, self.y, self.w, self.h, self.lx0, self.lz0, self.res, self.origin_x_m,
+                self.origin_z_m, self.cell_m)
 
 
 class WorldParams(C.Structure):
@@ -160,7 +177,7 @@ class HostWorldView(C.Structure):
                 ("grid_dims", C.c_uint32 * 3)]
 
 
-assert C.sizeof(Camera) == 464 and C.sizeof(Clipmap) == 528
+assert C.sizeof(Camera) == 464 and C.sizeof(Clipmap) == 528 and C.sizeof(ClipRect) == 48
 
 
 class VoxhipError(RuntimeError):
@@ -214,6 +231,13 @@ def _load() -> C.CDLL:
         "vx_camera_build": ([C.POINTER(Pose), u32, u32, u32, u32, u32, f32, f32, f32, C.POINTER(i32), C.POINTER(u32), u32,
                              vp, u32, C.POINTER(Camera)], C.c_int),
         "vx_clipmap_build": ([C.POINTER(WorldParams), f32, f32, u32, u32, f32, C.POINTER(Clipmap), vp], C.c_int),
+        "vx_clipmap_state_create": ([u32, u32, f32, f32, u32, C.POINTER(vp)], C.c_int),
+        "vx_clipmap_state_destroy": ([vp], None), "vx_clipmap_invalidate": ([vp], C.c_int),
+        "vx_clipmap_update": ([vp, f32, f32, f32, C.POINTER(Clipmap), C.POINTER(ClipRect), u32, C.POINTER(u32)],
+                              C.c_int),
+        "vx_clipmap_fill_rects": ([C.POINTER(WorldParams), C.POINTER(ClipRect), u32, vp], C.c_int),
+        "vx_write_clipmap_rects": ([vp, C.POINTER(Clipmap), C.POINTER(WorldParams), C.POINTER(ClipRect), u32], C.c_int),
+        "vx_read_clipmap": ([vp, vp, C.c_size_t], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -232,7 +256,8 @@ EXPORTED = (
     "vx_query_rays", "vx_query_timing", "vx_query_set_config", "vx_query_get_config", "vx_query_tune", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
     "vx_batch_set_config", "vx_batch_get_config", "vx_batch_tune", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
     "vx_host_world_build", "vx_host_world_free", "vx_host_world_get", "vx_host_world_edit", "vx_camera_build",
-    "vx_clipmap_build",
+    "vx_clipmap_build", "vx_clipmap_state_create", "vx_clipmap_state_destroy", "vx_clipmap_invalidate",
+    "vx_clipmap_update", "vx_clipmap_fill_rects", "vx_write_clipmap_rects", "vx_read_clipmap",
 )
 
 
@@ -383,6 +408,79 @@ def clipmap_build(params: WorldParams, cam_x: float, cam_z: float, levels: int =
     return clip, heights
 
 
+class ClipmapState:
+    """Clipmap (src/clipmap.rs:44-60) with its per-frame update
+    (clipmap.rs:182-577): committed origin cell and toroidal offset per level.
+    wrap_fix places a strip's spanning axis at its wrapped storage position
+    (VX_CLIP_WRAP_FIX); by default strips are placed as the reference does."""
+
+    def __init__(self, levels: int = 5, res: int = 256, base_cell_m: float = 1.0, min_update_interval_s: float = 0.0,
+                 wrap_fix: bool = False):
+        h = C.c_void_p()
+        _check(LIB.vx_clipmap_state_create(levels, res, base_cell_m, min_update_interval_s,
+                                           CLIP_WRAP_FIX if wrap_fix else 0, C.byref(h)), "vx_clipmap_state_create")
+        self._h = h
+        self.levels, self.res = levels, res
+        self._rects = (ClipRect * (4 * levels))()
+
+    def update(self, cam_x: float, cam_z: float, time_s: float) -> tuple[Clipmap, list]:
+        """Clipmap::update without the heights: (committed uniform, rectangles)."""
+        clip = Clipmap()
+        n = C.c_uint32()
+        _check(LIB.vx_clipmap_update(self._h, cam_x, cam_z, time_s, C.byref(clip), self._rects, len(self._rects),
+                                     C.byref(n)), "vx_clipmap_update")
+        return clip, [ClipRect.from_buffer_copy(self._rects[i]) for i in range(n.value)]
+
+    def invalidate(self) -> None:
+        _check(LIB.vx_clipmap_invalidate(self._h), "vx_clipmap_invalidate")
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_clipmap_state_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _rect_array(rects: list):
+    arr = (ClipRect * max(1, len(rects)))()
+    for i, r in enumerate(rects):
+        arr[i] = r
+    return arr
+
+
+def clipmap_fill_rects(params: WorldParams, rects: list) -> list:
+    """Heights of each rectangle on the host (ClipmapUpload::data_f32): a list
+    of (h, w) float32 arrays."""
+    total = sum(r.w * r.h for r in rects)
+    data = np.zeros(max(total, 1), np.float32)
+    _check(LIB.vx_clipmap_fill_rects(C.byref(params), _rect_array(rects), len(rects), _ptr(data)),
+           "vx_clipmap_fill_rects")
+    out, k = [], 0
+    for r in rects:
+        out.append(data[k:k + r.w * r.h].reshape(r.h, r.w))
+        k += r.w * r.h
+    return out
+
+
+def apply_clip_rects(heights: np.ndarray, rects: list, data: list) -> None:
+    """Writes filled rectangles into a (levels, res, res) storage array."""
+    for r, d in zip(rects, data):
+        heights[r.level, r.y:r.y + r.h, r.x:r.x + r.w] = d
+
+
+def clip_logical(clip: Clipmap, heights: np.ndarray, level: int) -> np.ndarray:
+    """A level of a storage array in logical order: logical (lx, lz) is
+    storage ((lx + off_x) mod res, (lz + off_z) mod res)."""
+    res = clip.res
+    ox, oz = clip.offset[level][0], clip.offset[level][1]
+    return np.roll(np.roll(heights[level], -int(oz), axis=0), -int(ox), axis=1)[:res, :res]
+
+
 # ---------------------------------------------------------------------------
 # Renderer boundary
 # ---------------------------------------------------------------------------
@@ -421,6 +519,24 @@ class Context:
         for lv in range(clip.levels):
             patches[lv] = ClipPatch(lv, 0, 0, clip.res, clip.res, h[lv].ctypes.data)
         _check(LIB.vx_write_clipmap(self._h, C.byref(clip), patches, clip.levels), "vx_write_clipmap")
+
+    def write_clipmap_patches(self, clip: Clipmap, rects: list, data: list) -> None:
+        """Renderer::write_clipmap_updates with host-built rectangles."""
+        keep = [np.ascontiguousarray(d, np.float32) for d in data]
+        patches = (ClipPatch * max(1, len(rects)))()
+        for i, (r, d) in enumerate(zip(rects, keep)):
+            patches[i] = ClipPatch(r.level, r.x, r.y, r.w, r.h, d.ctypes.data)
+        _check(LIB.vx_write_clipmap(self._h, C.byref(clip), patches, len(rects)), "vx_write_clipmap")
+
+    def write_clipmap_rects(self, clip: Clipmap, params: WorldParams, rects: list) -> None:
+        """Renderer::write_clipmap_updates with the heights generated on the device."""
+        _check(LIB.vx_write_clipmap_rects(self._h, C.byref(clip), C.byref(params), _rect_array(rects), len(rects)),
+               "vx_write_clipmap_rects")
+
+    def read_clipmap(self) -> np.ndarray:
+        out = np.zeros((self.clip_levels, self.clip_res, self.clip_res), np.float32)
+        _check(LIB.vx_read_clipmap(self._h, _ptr(out), out.nbytes), "vx_read_clipmap")
+        return out
 
     def sync(self) -> None:
         _check(LIB.vx_sync(self._h), "vx_sync")
