@@ -188,6 +188,9 @@ int vx_upload_regions(vx_ctx* ctx, int arena, const vx_region* regions, uint32_t
 /* Renderer::write_chunk_grid (render/state/mod.rs:310-317).  The grid origin and
  * dims travel in vx_camera, as in the reference; here only the cell values. */
 int vx_write_grid(vx_ctx* ctx, const uint32_t* grid, uint32_t cells);
+/* Copies count elements at an element offset of an arena back to host memory
+ * (no reference counterpart: for tests and tools). */
+int vx_read_region(vx_ctx* ctx, int arena, uint64_t offset, uint64_t count, void* dst);
 
 /* Renderer::write_clipmap_updates (render/state/mod.rs:586-634).  Patches are
  * rectangles of one level in storage (toroidal) texel coordinates. */
@@ -353,6 +356,29 @@ int vx_build_chunk(const uint8_t* material, const int32_t* ground, int32_t chunk
                    int32_t dirt_depth, vx_node* nodes, vx_node_ropes* ropes,
                    uint32_t node_cap, uint32_t* node_count,
                    uint32_t macro_words[16], uint32_t colinfo_words[2048]);
+
+/* The chunk builder on the device: for a batch of chunks, the same node,
+ * rope, macro and colinfo bytes as vx_build_chunk, built on the GPU straight
+ * into the context's resident arenas (no reference counterpart: the reference
+ * builds on CPU worker threads, svo/builder.rs:846-986).  Two steps:
+ *   vx_chunk_builder_count: material (n * 64^3 bytes, y-slab layout, as for
+ *     vx_build_chunk) and ground (n * 64^2 int32) are DEVICE arrays, chunk_oy a
+ *     host array; writes each chunk's node count to node_counts (host).  The
+ *     material must stay valid until the emit step returns.
+ *   vx_chunk_builder_emit: writes the counted chunks at the given element
+ *     offsets of the node and rope arenas and of the macro and colinfo words
+ *     (the ChunkMetaGpu node_base / macro_base / colinfo_base the host then
+ *     uploads).  Ranges past the context's capacities fail with VX_E_CAPACITY.
+ * Both complete before they return.  The workspace holds about 450 KB of HBM
+ * per chunk of max_chunks (at most VX_BUILD_CHUNKS_MAX). */
+typedef struct vx_chunk_builder vx_chunk_builder;
+#define VX_BUILD_CHUNKS_MAX 1024u
+int  vx_chunk_builder_create(vx_ctx* ctx, uint32_t max_chunks, vx_chunk_builder** out);
+void vx_chunk_builder_destroy(vx_chunk_builder* b);
+int  vx_chunk_builder_count(vx_chunk_builder* b, uint32_t n, const uint8_t* material, const int32_t* ground,
+                            const int32_t* chunk_oy, int32_t dirt_depth, uint32_t* node_counts);
+int  vx_chunk_builder_emit(vx_chunk_builder* b, const uint64_t* node_base, const uint32_t* macro_base,
+                           const uint32_t* colinfo_base);
 
 /* The seeded synthetic world: 2-octave value-noise heightfield, stone/dirt/grass
  * layers and trees with leaf blobs.  Its definition is this library's (the

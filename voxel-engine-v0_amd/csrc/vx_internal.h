@@ -28,3 +28,18 @@ int vx_fail(int code, const char* fmt, ...);
 // Context properties for the library's other translation units.
 uint64_t vx_internal_node_capacity(const vx_ctx* ctx);
 int vx_internal_device(const vx_ctx* ctx);
+
+// The resident arenas of a context, for the library's device-side producers.
+struct vx_internal_arenas {
+  vx_node* nodes;
+  vx_node_ropes* ropes;
+  uint32_t* macro;
+  uint32_t* colinfo;
+  uint64_t node_cap;
+  uint32_t chunk_cap;
+  void* stream;  // the context's hipStream_t
+  int device;
+};
+void vx_internal_get_arenas(const vx_ctx* ctx, vx_internal_arenas* out);
+// Marks the node arena as changed (per-slot derived flags are refreshed before the next render).
+void vx_internal_nodes_changed(vx_ctx* ctx);

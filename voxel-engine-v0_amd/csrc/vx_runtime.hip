@@ -68,6 +68,17 @@ struct vx_ctx {
 
 uint64_t vx_internal_node_capacity(const vx_ctx* ctx) { return ctx->node_cap; }
 int vx_internal_device(const vx_ctx* ctx) { return ctx->device; }
+void vx_internal_get_arenas(const vx_ctx* ctx, vx_internal_arenas* out) {
+  out->nodes = ctx->nodes;
+  out->ropes = ctx->ropes;
+  out->macro = ctx->macro;
+  out->colinfo = ctx->colinfo;
+  out->node_cap = ctx->node_cap;
+  out->chunk_cap = ctx->chunk_cap;
+  out->stream = (void*)ctx->stream;
+  out->device = ctx->device;
+}
+void vx_internal_nodes_changed(vx_ctx* ctx) { ctx->lights_dirty = true; }
 
 struct vx_view {
   vx_ctx* ctx = nullptr;
@@ -332,6 +343,29 @@ int vx_upload_regions(vx_ctx* ctx, int arena, const vx_region* regions, uint32_t
   }
   VX_HIP(hipStreamSynchronize(ctx->stream));
   if (arena == VX_ARENA_NODES || arena == VX_ARENA_META) ctx->lights_dirty = true;
+  return VX_OK;
+}
+
+int vx_read_region(vx_ctx* ctx, int arena, uint64_t offset, uint64_t count, void* dst) {
+  if (!ctx || (!dst && count)) return vx_fail(VX_E_INVALID, "vx_read_region: null argument");
+  const char* base = nullptr;
+  size_t esz = 0;
+  uint64_t cap = 0;
+  switch (arena) {
+    case VX_ARENA_NODES: base = (const char*)ctx->nodes; esz = sizeof(vx_node); cap = ctx->node_cap; break;
+    case VX_ARENA_ROPES: base = (const char*)ctx->ropes; esz = sizeof(vx_node_ropes); cap = ctx->node_cap; break;
+    case VX_ARENA_META: base = (const char*)ctx->meta; esz = sizeof(vx_chunk_meta); cap = ctx->chunk_cap; break;
+    case VX_ARENA_MACRO: base = (const char*)ctx->macro; esz = 4; cap = (uint64_t)ctx->chunk_cap * 16; break;
+    case VX_ARENA_COLINFO: base = (const char*)ctx->colinfo; esz = 4; cap = (uint64_t)ctx->chunk_cap * 2048; break;
+    default: return vx_fail(VX_E_INVALID, "vx_read_region: bad arena %d", arena);
+  }
+  if (offset > cap || count > cap - offset)
+    return vx_fail(VX_E_CAPACITY, "vx_read_region: [%llu, +%llu) exceeds capacity %llu", (unsigned long long)offset,
+                   (unsigned long long)count, (unsigned long long)cap);
+  if (!count) return VX_OK;
+  DeviceGuard g(ctx->device);
+  VX_HIP(hipMemcpyAsync(dst, base + offset * esz, count * esz, hipMemcpyDeviceToHost, ctx->stream));
+  VX_HIP(hipStreamSynchronize(ctx->stream));
   return VX_OK;
 }
 

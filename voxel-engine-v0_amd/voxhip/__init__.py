@@ -238,6 +238,11 @@ def _load() -> C.CDLL:
         "vx_clipmap_fill_rects": ([C.POINTER(WorldParams), C.POINTER(ClipRect), u32, vp], C.c_int),
         "vx_write_clipmap_rects": ([vp, C.POINTER(Clipmap), C.POINTER(WorldParams), C.POINTER(ClipRect), u32], C.c_int),
         "vx_read_clipmap": ([vp, vp, C.c_size_t], C.c_int),
+        "vx_read_region": ([vp, C.c_int, u64, u64, vp], C.c_int),
+        "vx_chunk_builder_create": ([vp, u32, C.POINTER(vp)], C.c_int),
+        "vx_chunk_builder_destroy": ([vp], None),
+        "vx_chunk_builder_count": ([vp, u32, vp, vp, C.POINTER(i32), i32, C.POINTER(u32)], C.c_int),
+        "vx_chunk_builder_emit": ([vp, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -257,7 +262,8 @@ EXPORTED = (
     "vx_batch_set_config", "vx_batch_get_config", "vx_batch_tune", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
     "vx_host_world_build", "vx_host_world_free", "vx_host_world_get", "vx_host_world_edit", "vx_camera_build",
     "vx_clipmap_build", "vx_clipmap_state_create", "vx_clipmap_state_destroy", "vx_clipmap_invalidate",
-    "vx_clipmap_update", "vx_clipmap_fill_rects", "vx_write_clipmap_rects", "vx_read_clipmap",
+    "vx_clipmap_update", "vx_clipmap_fill_rects", "vx_write_clipmap_rects", "vx_read_clipmap", "vx_read_region",
+    "vx_chunk_builder_create", "vx_chunk_builder_destroy", "vx_chunk_builder_count", "vx_chunk_builder_emit",
 )
 
 
@@ -533,6 +539,14 @@ class Context:
         _check(LIB.vx_write_clipmap_rects(self._h, C.byref(clip), C.byref(params), _rect_array(rects), len(rects)),
                "vx_write_clipmap_rects")
 
+    def read_region(self, arena: int, offset: int, count: int) -> np.ndarray:
+        """Elements [offset, offset + count) of an arena, copied to the host."""
+        dt = {ARENA_NODES: NODE_DTYPE, ARENA_ROPES: ROPES_DTYPE, ARENA_META: META_DTYPE, ARENA_MACRO: np.uint32,
+              ARENA_COLINFO: np.uint32}[arena]
+        out = np.zeros(count, dt)
+        _check(LIB.vx_read_region(self._h, arena, offset, count, out.ctypes.data), "vx_read_region")
+        return out
+
     def read_clipmap(self) -> np.ndarray:
         out = np.zeros((self.clip_levels, self.clip_res, self.clip_res), np.float32)
         _check(LIB.vx_read_clipmap(self._h, _ptr(out), out.nbytes), "vx_read_clipmap")
@@ -607,6 +621,48 @@ class View:
     def close(self) -> None:
         if self._h:
             LIB.vx_view_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ChunkBuilder:
+    """The chunk builder on the device (vx_chunk_builder_*): count a batch of
+    chunks whose material (n x 64^3 u8, y-slab) and ground (n x 64^2 i32) are
+    device arrays, then emit them into the context's arenas at the given bases."""
+
+    def __init__(self, ctx: Context, max_chunks: int):
+        h = C.c_void_p()
+        _check(LIB.vx_chunk_builder_create(ctx._h, max_chunks, C.byref(h)), "vx_chunk_builder_create")
+        self._h = h
+        self.ctx = ctx
+        self.max_chunks = max_chunks
+
+    def count(self, material: "DeviceArray", ground: "DeviceArray", chunk_oy, dirt_depth: int,
+              n: int | None = None) -> np.ndarray:
+        oy = np.ascontiguousarray(chunk_oy, np.int32)
+        n = len(oy) if n is None else n
+        counts = np.zeros(max(n, 1), np.uint32)
+        _check(LIB.vx_chunk_builder_count(self._h, n, material.ptr, ground.ptr,
+                                          oy.ctypes.data_as(C.POINTER(C.c_int32)), dirt_depth,
+                                          counts.ctypes.data_as(C.POINTER(C.c_uint32))), "vx_chunk_builder_count")
+        return counts[:n]
+
+    def emit(self, node_base, macro_base, colinfo_base) -> None:
+        nb = np.ascontiguousarray(node_base, np.uint64)
+        mb = np.ascontiguousarray(macro_base, np.uint32)
+        cb = np.ascontiguousarray(colinfo_base, np.uint32)
+        _check(LIB.vx_chunk_builder_emit(self._h, nb.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         mb.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         cb.ctypes.data_as(C.POINTER(C.c_uint32))), "vx_chunk_builder_emit")
+
+    def close(self) -> None:
+        if self._h:
+            LIB.vx_chunk_builder_destroy(self._h)
             self._h = None
 
     def __del__(self):
