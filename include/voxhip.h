@@ -542,13 +542,20 @@ typedef struct vx_server vx_server;
 typedef struct vx_server_config {
     uint64_t memory_budget;          /* bytes for resident view streams + in-flight ray batches */
     uint32_t max_views_per_launch;   /* default 64 */
-    uint32_t edit_threads;           /* host threads rebuilding edited chunks (default 8) */
+    uint32_t edit_threads;           /* host threads rebuilding edited chunks (default 8); with
+                                        VX_SERVER_DEVICE_BUILD, threads filling the resident material */
     uint32_t max_queued_frames;      /* per stream, queued + in flight; more is rejected (0: no limit) */
     float    latency_target_ms;      /* submit -> done target for admission and launch size (0: none) */
     vx_world_params world;           /* generator of the resident world */
     int32_t  grid_origin[3];
     uint32_t grid_dims[3];
+    uint32_t flags;                  /* VX_SERVER_* */
 } vx_server_config;
+/* Keep every slot's material resident in HBM (64^3 B + 16 KB per chunk, filled
+ * from the generator at create on edit_threads host threads) and rebuild edited
+ * chunks with the device chunk builder at launch boundaries, up to 256 chunks
+ * per boundary, instead of on host worker threads. */
+#define VX_SERVER_DEVICE_BUILD 1u
 #define VX_REQ_VIEW 1u
 #define VX_REQ_RAYS 2u
 typedef struct vx_request {

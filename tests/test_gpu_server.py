@@ -146,7 +146,8 @@ def test_latency_target_admission(gpu):
     srv.close()
 
 
-def test_edits_match_rebuilt_world(gpu):
+@pytest.mark.parametrize("device_build", [False, True])
+def test_edits_match_rebuilt_world(gpu, device_build):
     from voxhip import scenes
 
     sc, _ = host_world("c0")
@@ -176,7 +177,10 @@ def test_edits_match_rebuilt_world(gpu):
         mats.append(int(rng.integers(1, 6)))
     xyz, mats = np.array(xyz, np.int32), np.array(mats, np.uint8)
 
-    srv = gpu.Server(ctx, hw, memory_budget=8 << 30, edit_threads=4)
+    srv = gpu.Server(ctx, hw, memory_budget=8 << 30, edit_threads=4, device_build=device_build)
+    # in two calls, the second re-editing voxels of the first (the last edit of a voxel wins)
+    half = len(xyz) // 2
+    srv.edit(xyz[:half], np.full(half, 5, np.uint8))
     srv.edit(xyz, mats)
     srv.wait_edits()
     out = np.zeros((res[1], res[0], 4), np.float32)
@@ -197,7 +201,7 @@ def test_edits_match_rebuilt_world(gpu):
     view.write_camera(cam)
     view.render(gpu.PASS_FRAME)
     ref = view.read(gpu.OUT_OUTPUT)
-    print(f"[server] edits applied: {st['edits_applied']} chunks; image differs from the unedited world: "
+    print(f"[server] device_build={device_build} edits applied: {st['edits_applied']} chunks; image differs from the unedited world: "
           f"{not np.array_equal(ref, out) or True}", flush=True)
     assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
     ref_hits = gpu.query_rays(ref_ctx, qcam, rays)

@@ -122,18 +122,20 @@ __device__ inline uint32_t gmip_first(uint32_t k) {
 }
 
 struct CountArgs {
-  const uint8_t* material;  // n * 64^3, y-slab
-  const int32_t* ground;    // n * 64^2
+  const uint8_t* material;  // chunks of 64^3, y-slab
+  const int32_t* ground;    // chunks of 64^2
+  const uint32_t* index;    // n: batch chunk -> position in material/ground (null: identity)
   const int32_t* chunk_oy;  // n
   int32_t dirt_depth;
   uint32_t* ws;
-  uint32_t* counts;         // n
+  uint32_t* counts;         // 2n: node count, holds a solid voxel
 };
 
 __global__ __launch_bounds__(kThreads) void k_build_count(CountArgs a) {
   const uint32_t chunk = blockIdx.x, t = threadIdx.x;
-  const uint8_t* mat = a.material + (size_t)chunk * kVoxels;
-  const int32_t* gr = a.ground + (size_t)chunk * 4096u;
+  const uint32_t src = a.index ? a.index[chunk] : chunk;
+  const uint8_t* mat = a.material + (size_t)src * kVoxels;
+  const int32_t* gr = a.ground + (size_t)src * 4096u;
   uint32_t* ws = a.ws + (size_t)chunk * kWsStride;
   uint32_t* info = ws + kWsInfo;
   const int32_t oy = a.chunk_oy[chunk];
@@ -214,11 +216,15 @@ __global__ __launch_bounds__(kThreads) void k_build_count(CountArgs a) {
     base += total;
     if (t == 0) ws[kWsMeta + L + 2] = base;
   }
-  if (t == 0) a.counts[chunk] = base;
+  if (t == 0) {
+    a.counts[2 * chunk] = base;
+    a.counts[2 * chunk + 1] = info[0] & kExists;
+  }
 }
 
 struct EmitArgs {
   const uint8_t* material;
+  const uint32_t* index;
   uint32_t* ws;
   const uint64_t* node_base;     // n
   const uint32_t* macro_base;    // n
@@ -243,7 +249,7 @@ __device__ inline uint32_t descend_one(const vx_node* nodes, uint32_t nei, uint3
 
 __global__ __launch_bounds__(kThreads) void k_build_emit(EmitArgs a) {
   const uint32_t chunk = blockIdx.x, t = threadIdx.x;
-  const uint8_t* mat = a.material + (size_t)chunk * kVoxels;
+  const uint8_t* mat = a.material + (size_t)(a.index ? a.index[chunk] : chunk) * kVoxels;
   uint32_t* ws = a.ws + (size_t)chunk * kWsStride;
   const uint32_t* info = ws + kWsInfo;
   const uint32_t* scan = ws + kWsScan;
@@ -355,13 +361,21 @@ __global__ __launch_bounds__(kThreads) void k_build_emit(EmitArgs a) {
   }
 }
 
+// Edits: material[offset[i]] = value[i] (offsets distinct).
+__global__ __launch_bounds__(256) void k_scatter_u8(uint8_t* __restrict__ dst, const uint64_t* __restrict__ off,
+                                                    const uint8_t* __restrict__ val, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) dst[off[i]] = val[i];
+}
+
 }  // namespace
 
 struct vx_chunk_builder {
   vx_ctx* ctx = nullptr;
   uint32_t max_chunks = 0;
   uint32_t* ws = nullptr;        // max_chunks * kWsStride
-  uint32_t* counts = nullptr;    // max_chunks
+  uint32_t* counts = nullptr;    // 2 * max_chunks: node count, solid
+  uint32_t* index = nullptr;     // max_chunks (indexed batches)
   int32_t* oy = nullptr;         // max_chunks
   uint64_t* node_base = nullptr; // max_chunks
   uint32_t* macro_base = nullptr;
@@ -369,7 +383,8 @@ struct vx_chunk_builder {
   // The counted batch, pending its emit.
   uint32_t n = 0;
   const uint8_t* material = nullptr;
-  std::vector<uint32_t> host_counts;
+  bool indexed = false;
+  std::vector<uint32_t> host_counts;  // 2 per chunk
 };
 
 namespace {
@@ -413,10 +428,11 @@ int vx_chunk_builder_create(vx_ctx* ctx, uint32_t max_chunks, vx_chunk_builder**
   if (!b) return vx_fail(VX_E_NOMEM, "vx_chunk_builder_create: host allocation");
   b->ctx = ctx;
   b->max_chunks = max_chunks;
-  b->host_counts.assign(max_chunks, 0u);
+  b->host_counts.assign(2 * (size_t)max_chunks, 0u);
   Guard g(ar.device);
   int rc = VX_OK;
-  if ((rc = balloc(&b->ws, (size_t)max_chunks * kWsStride)) || (rc = balloc(&b->counts, max_chunks)) ||
+  if ((rc = balloc(&b->ws, (size_t)max_chunks * kWsStride)) || (rc = balloc(&b->counts, 2 * (size_t)max_chunks)) ||
+      (rc = balloc(&b->index, max_chunks)) ||
       (rc = balloc(&b->oy, max_chunks)) || (rc = balloc(&b->node_base, max_chunks)) ||
       (rc = balloc(&b->macro_base, max_chunks)) || (rc = balloc(&b->colinfo_base, max_chunks))) {
     vx_chunk_builder_destroy(b);
@@ -432,14 +448,17 @@ void vx_chunk_builder_destroy(vx_chunk_builder* b) {
   vx_internal_get_arenas(b->ctx, &ar);
   Guard g(ar.device);
   hipStreamSynchronize((hipStream_t)ar.stream);
-  for (void* p : {(void*)b->ws, (void*)b->counts, (void*)b->oy, (void*)b->node_base, (void*)b->macro_base,
+  for (void* p : {(void*)b->ws, (void*)b->counts, (void*)b->index, (void*)b->oy, (void*)b->node_base, (void*)b->macro_base,
                   (void*)b->colinfo_base})
     if (p) hipFree(p);
   delete b;
 }
 
-int vx_chunk_builder_count(vx_chunk_builder* b, uint32_t n, const uint8_t* material, const int32_t* ground,
-                           const int32_t* chunk_oy, int32_t dirt_depth, uint32_t* node_counts) {
+}  // extern "C"
+
+int vx_internal_builder_count(vx_chunk_builder* b, uint32_t n, const uint8_t* material, const int32_t* ground,
+                              const uint32_t* index, const int32_t* chunk_oy, int32_t dirt_depth, uint32_t* node_counts,
+                              uint8_t* solid) {
   if (!b || (n && (!material || !ground || !chunk_oy || !node_counts)))
     return vx_fail(VX_E_INVALID, "vx_chunk_builder_count: null argument");
   if (n > b->max_chunks) return vx_fail(VX_E_INVALID, "vx_chunk_builder_count: %u chunks > %u", n, b->max_chunks);
@@ -451,19 +470,39 @@ int vx_chunk_builder_count(vx_chunk_builder* b, uint32_t n, const uint8_t* mater
   hipStream_t st = (hipStream_t)ar.stream;
   Guard g(ar.device);
   VXB_HIP(hipMemcpyAsync(b->oy, chunk_oy, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  CountArgs a{material, ground, b->oy, dirt_depth, b->ws, b->counts};
+  if (index) VXB_HIP(hipMemcpyAsync(b->index, index, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  CountArgs a{material, ground, index ? b->index : nullptr, b->oy, dirt_depth, b->ws, b->counts};
   hipLaunchKernelGGL(k_build_count, dim3(n), dim3(kThreads), 0, st, a);
   VXB_HIP(hipGetLastError());
-  VXB_HIP(hipMemcpyAsync(b->host_counts.data(), b->counts, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  VXB_HIP(hipMemcpyAsync(b->host_counts.data(), b->counts, 2 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   VXB_HIP(hipStreamSynchronize(st));
   for (uint32_t i = 0; i < n; ++i) {
-    if (b->host_counts[i] == 0 || b->host_counts[i] > VX_MAX_CHUNK_NODES)
-      return vx_fail(VX_E_DEVICE, "vx_chunk_builder_count: chunk %u counted %u nodes", i, b->host_counts[i]);
-    node_counts[i] = b->host_counts[i];
+    const uint32_t c = b->host_counts[2 * i];
+    if (c == 0 || c > VX_MAX_CHUNK_NODES) return vx_fail(VX_E_DEVICE, "vx_chunk_builder_count: chunk %u counted %u nodes", i, c);
+    node_counts[i] = c;
+    if (solid) solid[i] = (uint8_t)(b->host_counts[2 * i + 1] != 0);
   }
   b->n = n;
   b->material = material;
+  b->indexed = index != nullptr;
   return VX_OK;
+}
+
+int vx_internal_scatter_u8(vx_ctx* ctx, uint8_t* dst, const uint64_t* d_off, const uint8_t* d_val, uint32_t n) {
+  if (!n) return VX_OK;
+  vx_internal_arenas ar;
+  vx_internal_get_arenas(ctx, &ar);
+  Guard g(ar.device);
+  hipLaunchKernelGGL(k_scatter_u8, dim3((n + 255u) / 256u), dim3(256), 0, (hipStream_t)ar.stream, dst, d_off, d_val, n);
+  VXB_HIP(hipGetLastError());
+  return VX_OK;
+}
+
+extern "C" {
+
+int vx_chunk_builder_count(vx_chunk_builder* b, uint32_t n, const uint8_t* material, const int32_t* ground,
+                           const int32_t* chunk_oy, int32_t dirt_depth, uint32_t* node_counts) {
+  return vx_internal_builder_count(b, n, material, ground, nullptr, chunk_oy, dirt_depth, node_counts, nullptr);
 }
 
 int vx_chunk_builder_emit(vx_chunk_builder* b, const uint64_t* node_base, const uint32_t* macro_base,
@@ -474,7 +513,7 @@ int vx_chunk_builder_emit(vx_chunk_builder* b, const uint64_t* node_base, const 
   vx_internal_get_arenas(b->ctx, &ar);
   const uint32_t n = b->n;
   for (uint32_t i = 0; i < n; ++i) {
-    const uint64_t cnt = b->host_counts[i];
+    const uint64_t cnt = b->host_counts[2 * i];
     if (node_base[i] > ar.node_cap || cnt > ar.node_cap - node_base[i])
       return vx_fail(VX_E_CAPACITY, "vx_chunk_builder_emit: chunk %u nodes [%llu, +%llu) exceed capacity %llu", i,
                      (unsigned long long)node_base[i], (unsigned long long)cnt, (unsigned long long)ar.node_cap);
@@ -487,7 +526,7 @@ int vx_chunk_builder_emit(vx_chunk_builder* b, const uint64_t* node_base, const 
   VXB_HIP(hipMemcpyAsync(b->node_base, node_base, n * sizeof(uint64_t), hipMemcpyHostToDevice, st));
   VXB_HIP(hipMemcpyAsync(b->macro_base, macro_base, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   VXB_HIP(hipMemcpyAsync(b->colinfo_base, colinfo_base, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-  EmitArgs a{b->material, b->ws, b->node_base, b->macro_base, b->colinfo_base, ar.nodes, ar.ropes, ar.macro, ar.colinfo};
+  EmitArgs a{b->material, b->indexed ? b->index : nullptr, b->ws, b->node_base, b->macro_base, b->colinfo_base, ar.nodes, ar.ropes, ar.macro, ar.colinfo};
   hipLaunchKernelGGL(k_build_emit, dim3(n), dim3(kThreads), 0, st, a);
   VXB_HIP(hipGetLastError());
   // Error flags of the batch (word 8 of each chunk's state).

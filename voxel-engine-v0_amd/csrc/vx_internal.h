@@ -43,3 +43,14 @@ struct vx_internal_arenas {
 void vx_internal_get_arenas(const vx_ctx* ctx, vx_internal_arenas* out);
 // Marks the node arena as changed (per-slot derived flags are refreshed before the next render).
 void vx_internal_nodes_changed(vx_ctx* ctx);
+
+// The device chunk builder over chunks at any positions of resident material
+// and ground arrays: batch chunk i reads position index[i] (host array; null:
+// position i).  solid (host, may be null) receives whether each chunk holds a
+// solid voxel.  The emit step (vx_chunk_builder_emit) reads the same positions.
+struct vx_chunk_builder;
+int vx_internal_builder_count(vx_chunk_builder* b, uint32_t n, const uint8_t* material, const int32_t* ground,
+                              const uint32_t* index, const int32_t* chunk_oy, int32_t dirt_depth, uint32_t* node_counts,
+                              uint8_t* solid);
+// dst[off[i]] = val[i] for device arrays off / val (distinct offsets), on the context stream.
+int vx_internal_scatter_u8(vx_ctx* ctx, uint8_t* dst, const uint64_t* d_off, const uint8_t* d_val, uint32_t n);

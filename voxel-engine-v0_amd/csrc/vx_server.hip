@@ -15,7 +15,11 @@
 //     fresh node-arena range (best-fit allocator), then its macro / colinfo /
 //     meta entries are rewritten and the old range is freed: the arena holds
 //     old and new copies until the swap (double-buffered replacement), and no
-//     launch is in flight at the boundary.
+//     launch is in flight at the boundary;
+//   * with VX_SERVER_DEVICE_BUILD the chunks' material stays resident in HBM
+//     instead: at a launch boundary the pending edits are scattered into it
+//     and the edited chunks are rebuilt by the device chunk builder straight
+//     into fresh arena ranges, then the same meta swap.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -171,6 +175,21 @@ struct vx_server {
   std::vector<double> lat_view, lat_rays;
   double t_start = 0.0;
   int last_error = VX_OK;
+  // Device builds (VX_SERVER_DEVICE_BUILD): per-slot material (64^3 B) and
+  // ground (64^2 i32) resident in HBM, the builder, pending edits (slot ->
+  // voxel index -> material, the last edit of a voxel wins), dirty slots in
+  // edit order, and device buffers for the edit scatter.
+  bool device_build = false;
+  uint8_t* dmat = nullptr;
+  int32_t* dground = nullptr;
+  vx_chunk_builder* builder = nullptr;
+  uint32_t build_batch = 0;
+  std::unordered_map<uint32_t, std::unordered_map<uint32_t, uint8_t>> pend;
+  std::deque<uint32_t> dev_q;
+  bool dev_busy = false;
+  uint64_t* d_off = nullptr;
+  uint8_t* d_val = nullptr;
+  size_t d_edit_cap = 0;
 };
 
 namespace {
@@ -305,6 +324,110 @@ int apply_ready(vx_server* s, std::vector<ChunkBuild>& builds) {
   return VX_OK;
 }
 
+// Launch boundary, device builds: scatter the batch's edits into the resident
+// material, count the edited chunks, place each in a fresh arena range, emit
+// them there, then swap their meta entries and release the old ranges.  When
+// the arena cannot place every chunk, the batch waits for the next boundary
+// (its edits are already in the material).
+int apply_device_builds(vx_server* s, const std::vector<uint32_t>& slots,
+                        std::unordered_map<uint32_t, std::unordered_map<uint32_t, uint8_t>>& edits, bool& requeue) {
+  requeue = false;
+  constexpr uint64_t kVox = 64ull * 64ull * 64ull;
+  std::vector<uint64_t> off;
+  std::vector<uint8_t> val;
+  for (auto& e : edits)
+    for (auto& v : e.second) {
+      off.push_back((uint64_t)e.first * kVox + v.first);
+      val.push_back(v.second);
+    }
+  vx_internal_arenas ar;
+  vx_internal_get_arenas(s->ctx, &ar);
+  hipStream_t st = (hipStream_t)ar.stream;
+  if (!off.empty()) {
+    if (off.size() > s->d_edit_cap) {
+      if (s->d_off) hipFree(s->d_off);
+      if (s->d_val) hipFree(s->d_val);
+      s->d_off = nullptr;
+      s->d_val = nullptr;
+      s->d_edit_cap = 0;
+      const size_t cap = std::max<size_t>(off.size(), 4096);
+      if (hipMalloc((void**)&s->d_off, cap * sizeof(uint64_t)) != hipSuccess ||
+          hipMalloc((void**)&s->d_val, cap) != hipSuccess)
+        return vx_fail(VX_E_NOMEM, "server: edit buffers");
+      s->d_edit_cap = cap;
+    }
+    if (hipMemcpyAsync(s->d_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(s->d_val, val.data(), val.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+      return vx_fail(VX_E_DEVICE, "server: edit upload");
+    int rc = vx_internal_scatter_u8(s->ctx, s->dmat, s->d_off, s->d_val, (uint32_t)off.size());
+    if (rc) return rc;
+  }
+  const uint32_t n = (uint32_t)slots.size();
+  std::vector<int32_t> oy(n);
+  for (uint32_t i = 0; i < n; ++i) oy[i] = s->meta[slots[i]].origin[1];
+  std::vector<uint32_t> counts(n);
+  std::vector<uint8_t> solid(n);
+  int rc = vx_internal_builder_count(s->builder, n, s->dmat, s->dground, slots.data(), oy.data(),
+                                     s->cfg.world.dirt_depth, counts.data(), solid.data());
+  if (rc) return rc;
+  std::vector<uint64_t> base(n);
+  std::vector<uint32_t> mb(n), cb(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!s->arena.alloc(counts[i], base[i])) {
+      for (uint32_t j = 0; j < i; ++j) s->arena.release(base[j], counts[j]);
+      requeue = true;
+      return VX_OK;
+    }
+    mb[i] = s->meta[slots[i]].macro_base;
+    cb[i] = s->meta[slots[i]].colinfo_base;
+  }
+  if ((rc = vx_chunk_builder_emit(s->builder, base.data(), mb.data(), cb.data()))) return rc;
+  std::vector<std::pair<uint64_t, uint64_t>> old;
+  std::vector<vx_region> mr;
+  for (uint32_t i = 0; i < n; ++i) {
+    vx_chunk_meta& m = s->meta[slots[i]];
+    old.emplace_back(m.node_base, m.node_count);
+    m.node_base = (uint32_t)base[i];
+    m.node_count = counts[i];
+    m.macro_empty = solid[i] ? 0u : 1u;
+    mr.push_back(vx_region{slots[i], 1, &m});
+  }
+  if ((rc = vx_upload_regions(s->ctx, VX_ARENA_META, mr.data(), (uint32_t)mr.size()))) return rc;
+  for (auto& o : old) s->arena.release(o.first, o.second);
+  return VX_OK;
+}
+
+// Fills the resident material and ground of every slot from the world
+// generator, on `threads` host threads, in batches uploaded to the device.
+int fill_device_world(vx_server* s, uint32_t threads) {
+  constexpr size_t kVox = 64ull * 64ull * 64ull;
+  const uint32_t slots = (uint32_t)s->meta.size(), batch = 128;
+  std::vector<uint8_t> mat(batch * kVox);
+  std::vector<int32_t> ground((size_t)batch * 4096);
+  for (uint32_t s0 = 0; s0 < slots; s0 += batch) {
+    const uint32_t nb = std::min(batch, slots - s0);
+    std::atomic<uint32_t> next{0};
+    std::atomic<int> err{VX_OK};
+    auto work = [&]() {
+      for (uint32_t i; (i = next.fetch_add(1)) < nb;) {
+        const vx_chunk_meta& m = s->meta[s0 + i];
+        const int rc = vx_world_fill_chunk(&s->cfg.world, m.origin[0] / 64, m.origin[1] / 64, m.origin[2] / 64,
+                                           mat.data() + i * kVox, ground.data() + (size_t)i * 4096);
+        if (rc) err.store(rc);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    if (err.load()) return err.load();
+    if (hipMemcpy(s->dmat + (size_t)s0 * kVox, mat.data(), nb * kVox, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->dground + (size_t)s0 * 4096, ground.data(), (size_t)nb * 4096 * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return vx_fail(VX_E_DEVICE, "vx_server_create: material upload");
+  }
+  return VX_OK;
+}
+
 double pct(std::vector<double> v, double p) {
   if (v.empty()) return 0.0;
   std::sort(v.begin(), v.end());
@@ -326,11 +449,28 @@ void device_main(vx_server* s) {
   for (;;) {
     std::vector<Pending> views, rays;
     std::vector<ChunkBuild> builds;
+    std::vector<uint32_t> dev_slots;
+    std::vector<uint64_t> dev_gens;
+    std::unordered_map<uint32_t, std::unordered_map<uint32_t, uint8_t>> dev_edits;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv.wait(lk, [&] { return s->stop || !s->inbox.empty() || !s->ready.empty(); });
+      s->cv.wait(lk, [&] { return s->stop || !s->inbox.empty() || !s->ready.empty() || !s->dev_q.empty(); });
       if (s->stop) break;
       builds.swap(s->ready);
+      // Device builds: the oldest dirty slots, up to one builder batch, with their pending edits.
+      while (!s->dev_q.empty() && dev_slots.size() < s->build_batch) {
+        const uint32_t slot = s->dev_q.front();
+        s->dev_q.pop_front();
+        s->dirty[slot] = 0;
+        dev_slots.push_back(slot);
+        dev_gens.push_back(s->gen[slot]);
+        auto it = s->pend.find(slot);
+        if (it != s->pend.end()) {
+          dev_edits[slot] = std::move(it->second);
+          s->pend.erase(it);
+        }
+      }
+      s->dev_busy = !dev_slots.empty();
       // Queued requests join this launch in arrival order: at most one frame
       // per view stream, max_views_per_launch views and, with a latency
       // target, half the target's worth of rays (at least one request); the
@@ -366,6 +506,23 @@ void device_main(vx_server* s) {
       s->inflight_t0 = now_ms();
     }
     int rc = builds.empty() ? VX_OK : apply_ready(s, builds);
+    if (!rc && !dev_slots.empty()) {
+      bool requeue = false;
+      rc = apply_device_builds(s, dev_slots, dev_edits, requeue);
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (!rc && requeue) {  // the arena is full: these chunks wait for the next boundary
+        for (uint32_t slot : dev_slots)
+          if (!s->dirty[slot]) {
+            s->dirty[slot] = 1;
+            s->dev_q.push_back(slot);
+          }
+      } else if (!rc) {
+        for (size_t i = 0; i < dev_slots.size(); ++i)
+          s->applied[dev_slots[i]] = std::max(s->applied[dev_slots[i]], dev_gens[i]);
+        s->st.edits_applied += dev_slots.size();
+      }
+      s->dev_busy = false;
+    }
     const double t0 = now_ms();
     // Views: bind each request to its stream's view and write the camera.
     std::vector<vx_view*> vs;
@@ -494,8 +651,24 @@ int vx_server_create(vx_ctx* ctx, const vx_host_world_view* world, const vx_serv
     delete s;
     return vx_fail(VX_E_DEVICE, "vx_server_create: query stream");
   }
+  s->device_build = (s->cfg.flags & VX_SERVER_DEVICE_BUILD) != 0;
+  if (s->device_build) {
+    const size_t slots = s->meta.size();
+    s->build_batch = (uint32_t)std::min<size_t>(256, slots);
+    int rc = VX_OK;
+    if (hipMalloc((void**)&s->dmat, slots * 64ull * 64ull * 64ull) != hipSuccess ||
+        hipMalloc((void**)&s->dground, slots * 4096ull * sizeof(int32_t)) != hipSuccess)
+      rc = vx_fail(VX_E_NOMEM, "vx_server_create: resident material (%zu slots)", slots);
+    if (!rc) rc = vx_chunk_builder_create(ctx, s->build_batch, &s->builder);
+    if (!rc) rc = fill_device_world(s, s->cfg.edit_threads);
+    if (rc) {
+      vx_server_destroy(s);
+      return rc;
+    }
+  }
   s->t_start = now_ms();
-  for (uint32_t i = 0; i < s->cfg.edit_threads; ++i) s->workers.emplace_back(worker_main, s);
+  if (!s->device_build)
+    for (uint32_t i = 0; i < s->cfg.edit_threads; ++i) s->workers.emplace_back(worker_main, s);
   s->device = std::thread(device_main, s);
   *out = s;
   return VX_OK;
@@ -514,6 +687,9 @@ void vx_server_destroy(vx_server* s) {
     if (w.joinable()) w.join();
   for (auto& kv : s->streams)
     if (kv.second.view) vx_view_destroy(kv.second.view);
+  if (s->builder) vx_chunk_builder_destroy(s->builder);
+  for (void* p : {(void*)s->dmat, (void*)s->dground, (void*)s->d_off, (void*)s->d_val})
+    if (p) hipFree(p);
   if (s->qstream) hipStreamDestroy(s->qstream);
   delete s;
 }
@@ -609,8 +785,16 @@ int vx_server_edit(vx_server* s, const int32_t* xyz, const uint8_t* mats, uint32
       uint32_t idx = 0;
       const uint32_t slot = slot_of_voxel(s, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], idx);
       if (slot == VX_INVALID_U32) continue;
-      s->edits[slot][idx] = mats[i];
       ++s->gen[slot];
+      if (s->device_build) {
+        s->pend[slot][idx] = mats[i];
+        if (!s->dirty[slot]) {
+          s->dirty[slot] = 1;
+          s->dev_q.push_back(slot);
+        }
+        continue;
+      }
+      s->edits[slot][idx] = mats[i];
       if (!s->dirty[slot]) {
         s->dirty[slot] = 1;
         s->rebuild_q.push_back(slot);
@@ -618,6 +802,7 @@ int vx_server_edit(vx_server* s, const int32_t* xyz, const uint8_t* mats, uint32
     }
   }
   s->wcv.notify_all();
+  s->cv.notify_all();
   return VX_OK;
 }
 
@@ -654,7 +839,7 @@ int vx_server_wait_edits(vx_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   // every queued edit has been built and applied at a launch boundary
   s->cv.wait(lk, [&] {
-    if (!s->rebuild_q.empty() || !s->ready.empty() || s->building) return false;
+    if (!s->rebuild_q.empty() || !s->ready.empty() || s->building || !s->dev_q.empty() || s->dev_busy) return false;
     for (size_t i = 0; i < s->gen.size(); ++i)
       if (s->applied[i] < s->gen[i]) return false;
     return true;

@@ -49,6 +49,7 @@ CTX_GODRAY_MACRO_FIX = 1
 CTX_PRIMARY_STAGED = 2
 CAM_JITTER = 1
 CLIP_WRAP_FIX = 1
+SERVER_DEVICE_BUILD = 1
 CLIP_EDGE_GUARD = 16
 CLIP_RECTS_MAX = 64
 
@@ -146,7 +147,7 @@ class QueryConfig(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_uint64), ("max_views_per_launch", C.c_uint32), ("edit_threads", C.c_uint32),
                 ("max_queued_frames", C.c_uint32), ("latency_target_ms", C.c_float), ("world", WorldParams),
-                ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_uint32 * 3)]
+                ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_uint32 * 3), ("flags", C.c_uint32)]
 
 
 class Request(C.Structure):
@@ -794,8 +795,10 @@ class Server:
     and ray batches over one context, with admission and an edit stream."""
 
     def __init__(self, ctx: "Context", world: "HostWorld", memory_budget: int, max_views_per_launch: int = 64,
-                 edit_threads: int = 8, max_queued_frames: int = 0, latency_target_ms: float = 0.0):
+                 edit_threads: int = 8, max_queued_frames: int = 0, latency_target_ms: float = 0.0,
+                 device_build: bool = False):
         cfg = ServerConfig()
+        cfg.flags = SERVER_DEVICE_BUILD if device_build else 0
         cfg.memory_budget = memory_budget
         cfg.max_views_per_launch = max_views_per_launch
         cfg.edit_threads = edit_threads
