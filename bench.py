@@ -53,6 +53,7 @@ def parse():
                     help="duration of the serving workload's arrival schedule (0: skip it)")
     ap.add_argument("--single-batch", action="store_true",
                     help="render the 4K view through a one-view batch (work-stealing launches) instead of vx_render")
+    ap.add_argument("--no-chunk-builds", action="store_true", help="skip the chunk builder measurement")
     ap.add_argument("--static-clipmap", action="store_true",
                     help="keep frame 0's clipmap instead of updating it every frame")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline)")
@@ -218,9 +219,12 @@ def main():
     batched = None if a.batch_views <= 0 else batching(vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps)
     queries = None if a.query_batches <= 0 else ray_queries(vx, scenes, a.query_batches)
     serving = None if a.serve_seconds <= 0 else serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms)
-    serving_untargeted = None
+    serving_untargeted = serving_device = None
     if a.serve_seconds > 0 and a.serve_target_ms > 0:
         serving_untargeted = serve(vx, scenes, a.serve_seconds, target_ms=0.0)
+    if a.serve_seconds > 0:
+        serving_device = serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms, device_build=True)
+    builds = None if a.no_chunk_builds else chunk_builds(vx, scenes)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -256,6 +260,8 @@ def main():
         "ray_queries": queries,
         "serving": serving,
         "serving_without_latency_target": serving_untargeted,
+        "serving_device_builds": serving_device,
+        "chunk_builds": builds,
         "setup_s": round(setup_s, 2),
     }
     if counts is not None:
@@ -433,7 +439,8 @@ def ray_queries(vx, scenes, n_batches):
             "tuned": tuned}
 
 
-def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budget=64 << 30, target_ms=100.0):
+def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budget=64 << 30, target_ms=100.0,
+          device_build=False):
     """The serving workload: Poisson arrivals (rate req/s, seed 5) for `seconds`,
     half render frames of n_streams client view streams (resolutions drawn
     as in the batching workload, each stream on its own seeded flight, at most
@@ -442,7 +449,9 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
     with a concurrent stream of edit_rate random single-voxel edits/s, and
     admission against a `budget`-byte device memory budget and, when
     target_ms > 0, against that latency target (the server also bounds each
-    launch to half the target's worth of rays)."""
+    launch to half the target's worth of rays).  device_build: edited chunks
+    are rebuilt by the device chunk builder from HBM-resident material instead
+    of on host worker threads."""
     import threading
 
     rng = np.random.default_rng(5)
@@ -457,7 +466,8 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
     res = [sizes[i] for i in rng.integers(0, len(sizes), n_streams)]
     qcam = vx.camera_build(vx.make_pose((1.0, 1.0, 1.0), 0.0, 0.0), (64, 64), (64, 64), 0, grid_origin=sc.grid_origin,
                            grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
-    srv = vx.Server(ctx, hw, memory_budget=budget, max_queued_frames=1, latency_target_ms=target_ms)
+    srv = vx.Server(ctx, hw, memory_budget=budget, max_queued_frames=1, latency_target_ms=target_ms,
+                    device_build=device_build)
     pool = scenes.query_rays(rng, 2_000_000, sc.grid_origin, sc.grid_dims, tmax_m=51.2)  # batches are slices of it
     # arrival schedule
     t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5)))
@@ -480,6 +490,7 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
 
     editor = threading.Thread(target=edits, daemon=True)
     t0 = time.perf_counter()
+    cpu0 = time.process_time()
     editor.start()
     submitted = rejected = 0
     view_rays = ray_rays = 0
@@ -509,19 +520,82 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
     editor.join()
     srv.drain()
     wall = time.perf_counter() - t0
+    cpu_s = time.process_time() - cpu0
+    srv.wait_edits()
     st = srv.stats()
     srv.close()
     return {"workload": f"Poisson {rate:g} req/s for {seconds:g} s (seed 5), half frames of {n_streams} view streams "
                         f"(320x180..3840x2160, at most 1 pending frame each), half ray batches (1e3..1e6 rays), "
                         f"{edit_rate:g} single-voxel edits/s, 32x4x32 chunks + 5-level clipmap, {budget >> 30} GB budget, "
-                        + (f"{target_ms:g} ms latency target" if target_ms > 0 else "no latency target"),
+                        + (f"{target_ms:g} ms latency target" if target_ms > 0 else "no latency target")
+                        + (", edited chunks rebuilt on the device" if device_build else
+                           ", edited chunks rebuilt on 8 host threads"),
             "requests": submitted, "rejected": rejected, "rejected_rate": round(rejected / max(1, submitted), 4),
             "completed": int(st["completed"]), "launches": int(st["launches"]),
             "rays_per_s": round((view_rays + ray_rays) / wall, 1), "view_rays": view_rays, "query_rays": ray_rays,
             "view_latency_ms": {"p50": round(st["view_p50_ms"], 2), "p99": round(st["view_p99_ms"], 2)},
             "rays_latency_ms": {"p50": round(st["rays_p50_ms"], 2), "p99": round(st["rays_p99_ms"], 2)},
             "edits_applied_chunks": int(st["edits_applied"]), "gpu_busy_frac": round(st["busy_ms"] / (wall * 1e3), 3),
-            "wall_s": round(wall, 2)}
+            "host_cpu_s": round(cpu_s, 2), "wall_s": round(wall, 2)}
+
+
+def chunk_builds(vx, scenes, repeats=3, cpu_sample=24):
+    """Chunk builds of the 16x4x16-chunk world (1024 chunks, materials filled
+    on the host beforehand and resident in HBM): the device builder's count +
+    emit for the whole batch, against the host builder (vx_build_chunk) on one
+    core over a sample of the same chunks."""
+    import ctypes as C
+
+    sc = scenes.c1()
+    p = sc.params
+    gx, gy, gz = sc.grid_dims
+    coords = [(x, y, z) for z in range(gz) for y in range(gy) for x in range(gx)]
+    mats, grounds = [], []
+    for cx, cy, cz in coords:
+        m, g = vx.fill_chunk(p, cx, cy, cz)
+        mats.append(m)
+        grounds.append(g)
+    n = len(coords)
+    oy = np.array([cy * 64 for _, cy, _ in coords], np.int32)
+    d_mat = vx.DeviceArray.from_numpy(np.concatenate(mats))
+    d_ground = vx.DeviceArray.from_numpy(np.concatenate(grounds))
+    probe = vx.Context(node_capacity=16, chunk_capacity=1, grid_capacity=1)
+    b = vx.ChunkBuilder(probe, max_chunks=n)
+    counts = b.count(d_mat, d_ground, oy, p.dirt_depth)
+    b.close()
+    probe.close()
+    total = int(counts.sum())
+    ctx = vx.Context(node_capacity=total, chunk_capacity=n, grid_capacity=1)
+    b = vx.ChunkBuilder(ctx, max_chunks=n)
+    base = np.concatenate([[0], np.cumsum(counts[:-1].astype(np.uint64))]).astype(np.uint64)
+    slots = np.arange(n, dtype=np.uint32)
+    times = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        b.count(d_mat, d_ground, oy, p.dirt_depth)
+        b.emit(base, slots * 16, slots * 2048)
+        times.append(time.perf_counter() - t)
+    gpu_s = float(np.median(times))
+    b.close()
+    ctx.close()
+    # host builder, one core
+    cap = vx.MAX_CHUNK_NODES
+    nodes = np.zeros(cap, vx.NODE_DTYPE)
+    ropes = np.zeros(cap, vx.ROPES_DTYPE)
+    macro = np.zeros(16, np.uint32)
+    colinfo = np.zeros(2048, np.uint32)
+    cnt = C.c_uint32()
+    pick = np.linspace(0, n - 1, cpu_sample).astype(int)
+    t = time.perf_counter()
+    for i in pick:
+        vx.LIB.vx_build_chunk(mats[i].ctypes.data, grounds[i].ctypes.data, int(oy[i]), p.dirt_depth, nodes.ctypes.data,
+                              ropes.ctypes.data, cap, C.byref(cnt), macro.ctypes.data, colinfo.ctypes.data)
+    cpu_s = (time.perf_counter() - t) / len(pick)
+    return {"workload": f"{n} chunks of the seed-42 16x4x16 world ({total} nodes), materials resident in HBM",
+            "device_ms": round(gpu_s * 1e3, 3), "device_chunks_per_s": round(n / gpu_s, 1),
+            "device_nodes_per_s": round(total / gpu_s, 1),
+            "host_1core_ms_per_chunk": round(cpu_s * 1e3, 3), "host_1core_chunks_per_s": round(1.0 / cpu_s, 1),
+            "host_sample_chunks": int(len(pick))}
 
 
 def cpu_baseline(sc, hw, clip, heights, cam, W, H, rows):

@@ -41,10 +41,11 @@ constexpr uint32_t kVoxels = 64u * 64u * 64u;
 // First cell of each level 0..5 in a chunk's per-level arrays (side 2^L).
 __host__ __device__ constexpr uint32_t level_first(uint32_t L) { return L == 0 ? 0u : level_first(L - 1) + (1u << (3 * (L - 1))); }
 constexpr uint32_t kCells = level_first(6);  // 37449
-// Workspace per chunk (u32): info, scan and node index per cell, then 16 words of
-// chunk state: base[0..7] (base[7] = node count), error flag.
+// Workspace per chunk (u32): info, scan and node index per cell, 16 words of
+// chunk state (base[0..7], base[7] = node count), then the 2048 colinfo words.
 constexpr uint32_t kWsInfo = 0, kWsScan = kCells, kWsIdx = 2 * kCells, kWsMeta = 3 * kCells;
-constexpr uint32_t kWsStride = (3 * kCells + 16 + 63) & ~63u;
+constexpr uint32_t kWsCol = (3 * kCells + 16 + 63) & ~63u;
+constexpr uint32_t kWsStride = kWsCol + 2048u;
 
 // Cell info word.
 constexpr uint32_t kExists = 1u, kLeaf = 2u, kFull = 4u;
@@ -143,6 +144,8 @@ __global__ __launch_bounds__(kThreads) void k_build_count(CountArgs a) {
   // Ground minimum mip (svo/mips.rs:11-113), levels of cube size 2..64.
   __shared__ int32_t gmin[1365];
   __shared__ uint32_t sh16[16];
+  __shared__ uint32_t top[4096];  // per column (z*64 + x): ((y+1) << 8 | material) of its highest solid voxel, 0 if none
+  for (uint32_t c = t; c < 4096u; c += kThreads) top[c] = 0u;
   {
     const uint32_t x = t & 31u, z = t >> 5;
     const int32_t* r0 = gr + (2 * z) * 64 + 2 * x;
@@ -176,6 +179,12 @@ __global__ __launch_bounds__(kThreads) void k_build_count(CountArgs a) {
           const uint32_t vx = 2 * x + (ci & 1u), vy = 2 * y + ((ci >> 1) & 1u), vz = 2 * z + (ci >> 2);
           ch[ci] = voxel_info(mat[(vy << 12) + (vz << 6) + vx]);
         }
+        // Column tops: the upper voxel of each of the cell's four columns wins over the lower.
+        for (uint32_t q = 0; q < 4; ++q) {
+          const uint32_t lo = ch[(q & 1u) | ((q >> 1) << 2)], hi = ch[(q & 1u) | 2u | ((q >> 1) << 2)];
+          const uint32_t e = hi ? ((2 * y + 2) << 8) | info_mat(hi) : lo ? ((2 * y + 1) << 8) | info_mat(lo) : 0u;
+          if (e) atomicMax(&top[((2 * z + (q >> 1)) << 6) + 2 * x + (q & 1u)], e);
+        }
       } else {
         for (uint32_t ci = 0; ci < 8; ++ci)
           ch[ci] = child[cell_index(bits + 1, 2 * x + (ci & 1u), 2 * y + ((ci >> 1) & 1u), 2 * z + (ci >> 2))];
@@ -187,13 +196,24 @@ __global__ __launch_bounds__(kThreads) void k_build_count(CountArgs a) {
     __syncthreads();
   }
 
+  // Colinfo words (svo/builder.rs:742-805): per column the highest solid
+  // voxel's y and material (y = 255, material 0 for an empty column), two
+  // columns per word, low half first.
+  for (uint32_t w = t; w < 2048u; w += kThreads) {
+    uint32_t word = 0;
+    for (uint32_t h = 0; h < 2; ++h) {
+      const uint32_t e = top[2 * w + h];
+      word |= (e ? ((e >> 8) - 1u) | ((e & 0xFFu) << 8) : 255u) << (16 * h);
+    }
+    ws[kWsCol + w] = word;
+  }
+
   // Per level, the scan of interior cells' child counts: first child's index in
   // the next level's packed list.  base[L+1] = base[L] + level L's total.
   uint32_t base = 1;  // node 0 is the root
   if (t == 0) {
     ws[kWsMeta + 0] = 0;
     ws[kWsMeta + 1] = 1;
-    ws[kWsMeta + 8] = 0;  // error flag
   }
   for (uint32_t L = 0; L <= 5; ++L) {
     const uint32_t ncell = 1u << (3 * L);
@@ -233,6 +253,7 @@ struct EmitArgs {
   vx_node_ropes* ropes;
   uint32_t* macro;
   uint32_t* colinfo;
+  uint32_t* err;  // set when a chunk's workspace is inconsistent
 };
 
 // Rope target one level down (builder_ropes.rs:31-75): an invalid rope stays
@@ -298,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void k_build_emit(EmitArgs a) {
       for (uint32_t ci = 0; ci < 8; ++ci) {
         if (kid[ci] == kInvalid) continue;
         if (kid[ci] >= total) {  // cannot happen for a consistent workspace; never write past the chunk
-          ws[kWsMeta + 8] = 1u;
+          atomicOr(a.err, 1u);
           continue;
         }
         const uint32_t hx = ci & 1u, hy = (ci >> 1) & 1u, hz = ci >> 2;
@@ -329,25 +350,9 @@ __global__ __launch_bounds__(kThreads) void k_build_emit(EmitArgs a) {
     __syncthreads();
   }
 
-  // Column tops (svo/builder.rs:742-805): per column the highest solid voxel's
-  // y and material (y = 255, material 0 for an empty column), two columns per word.
+  // Colinfo words, computed by the count step.
   uint32_t* col = a.colinfo + a.colinfo_base[chunk];
-  for (uint32_t w = t; w < 2048u; w += kThreads) {
-    uint32_t word = 0;
-    for (uint32_t h = 0; h < 2; ++h) {
-      const uint32_t c = 2 * w + h;
-      uint32_t e = 255u;
-      for (int y = 63; y >= 0; --y) {
-        const uint32_t m = mat[((uint32_t)y << 12) + c];
-        if (m) {
-          e = (uint32_t)y | (m << 8);
-          break;
-        }
-      }
-      word |= e << (16 * h);
-    }
-    col[w] = word;
-  }
+  for (uint32_t w = t; w < 2048u; w += kThreads) col[w] = ws[kWsCol + w];
   // Macro occupancy (svo/builder.rs:809-841): bit mx + 8*(my + 8*mz) for each
   // 8^3-voxel cell holding a solid voxel, i.e. each present level-3 cell.
   if (t < 16) {
@@ -380,6 +385,7 @@ struct vx_chunk_builder {
   uint64_t* node_base = nullptr; // max_chunks
   uint32_t* macro_base = nullptr;
   uint32_t* colinfo_base = nullptr;
+  uint32_t* err = nullptr;
   // The counted batch, pending its emit.
   uint32_t n = 0;
   const uint8_t* material = nullptr;
@@ -434,7 +440,8 @@ int vx_chunk_builder_create(vx_ctx* ctx, uint32_t max_chunks, vx_chunk_builder**
   if ((rc = balloc(&b->ws, (size_t)max_chunks * kWsStride)) || (rc = balloc(&b->counts, 2 * (size_t)max_chunks)) ||
       (rc = balloc(&b->index, max_chunks)) ||
       (rc = balloc(&b->oy, max_chunks)) || (rc = balloc(&b->node_base, max_chunks)) ||
-      (rc = balloc(&b->macro_base, max_chunks)) || (rc = balloc(&b->colinfo_base, max_chunks))) {
+      (rc = balloc(&b->macro_base, max_chunks)) || (rc = balloc(&b->colinfo_base, max_chunks)) ||
+      (rc = balloc(&b->err, 1))) {
     vx_chunk_builder_destroy(b);
     return rc;
   }
@@ -449,7 +456,7 @@ void vx_chunk_builder_destroy(vx_chunk_builder* b) {
   Guard g(ar.device);
   hipStreamSynchronize((hipStream_t)ar.stream);
   for (void* p : {(void*)b->ws, (void*)b->counts, (void*)b->index, (void*)b->oy, (void*)b->node_base, (void*)b->macro_base,
-                  (void*)b->colinfo_base})
+                  (void*)b->colinfo_base, (void*)b->err})
     if (p) hipFree(p);
   delete b;
 }
@@ -526,19 +533,18 @@ int vx_chunk_builder_emit(vx_chunk_builder* b, const uint64_t* node_base, const 
   VXB_HIP(hipMemcpyAsync(b->node_base, node_base, n * sizeof(uint64_t), hipMemcpyHostToDevice, st));
   VXB_HIP(hipMemcpyAsync(b->macro_base, macro_base, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   VXB_HIP(hipMemcpyAsync(b->colinfo_base, colinfo_base, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-  EmitArgs a{b->material, b->indexed ? b->index : nullptr, b->ws, b->node_base, b->macro_base, b->colinfo_base, ar.nodes, ar.ropes, ar.macro, ar.colinfo};
+  VXB_HIP(hipMemsetAsync(b->err, 0, sizeof(uint32_t), st));
+  EmitArgs a{b->material, b->indexed ? b->index : nullptr, b->ws, b->node_base, b->macro_base, b->colinfo_base,
+             ar.nodes, ar.ropes, ar.macro, ar.colinfo, b->err};
   hipLaunchKernelGGL(k_build_emit, dim3(n), dim3(kThreads), 0, st, a);
   VXB_HIP(hipGetLastError());
-  // Error flags of the batch (word 8 of each chunk's state).
-  std::vector<uint32_t> flags(n);
-  VXB_HIP(hipMemcpy2DAsync(flags.data(), sizeof(uint32_t), b->ws + kWsMeta + 8, (size_t)kWsStride * sizeof(uint32_t),
-                           sizeof(uint32_t), n, hipMemcpyDeviceToHost, st));
+  uint32_t err = 0;
+  VXB_HIP(hipMemcpyAsync(&err, b->err, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   VXB_HIP(hipStreamSynchronize(st));
   vx_internal_nodes_changed(b->ctx);
   b->n = 0;
   b->material = nullptr;
-  for (uint32_t i = 0; i < n; ++i)
-    if (flags[i]) return vx_fail(VX_E_DEVICE, "vx_chunk_builder_emit: chunk %u: inconsistent workspace", i);
+  if (err) return vx_fail(VX_E_DEVICE, "vx_chunk_builder_emit: inconsistent workspace");
   return VX_OK;
 }
 
