@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--single-batch", action="store_true",
                     help="render the 4K view through a one-view batch (work-stealing launches) instead of vx_render")
     ap.add_argument("--no-chunk-builds", action="store_true", help="skip the chunk builder measurement")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed views + ray batches workload")
+    ap.add_argument("--only", choices=["mixed", "chunk_builds", "ray_queries", "serving", "serving_device_builds"],
+                    help="run one workload alone and print its result (not the benchmark line)")
     ap.add_argument("--static-clipmap", action="store_true",
                     help="keep frame 0's clipmap instead of updating it every frame")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline)")
@@ -126,6 +129,16 @@ def main():
         sys.exit(2)
     import voxhip as vx
     from voxhip import scenes
+
+    if a.only:
+        # One workload alone, its JSON object on one line (for profiling a leg).
+        legs = {"mixed": lambda: mixed(vx, scenes), "chunk_builds": lambda: chunk_builds(vx, scenes),
+                "ray_queries": lambda: ray_queries(vx, scenes, a.query_batches),
+                "serving": lambda: serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms),
+                "serving_device_builds": lambda: serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms,
+                                                       device_build=True)}
+        print(json.dumps({a.only: legs[a.only]()}), flush=True)
+        return
 
     W, H = (int(v) for v in a.res.lower().split("x"))
     t0 = time.time()
@@ -218,6 +231,7 @@ def main():
     clipmap = None if a.static_clipmap else clipmap_costs(vx, sc, ctx)
     batched = None if a.batch_views <= 0 else batching(vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps)
     queries = None if a.query_batches <= 0 else ray_queries(vx, scenes, a.query_batches)
+    mix = None if a.no_mixed else mixed(vx, scenes)
     serving = None if a.serve_seconds <= 0 else serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms)
     serving_untargeted = serving_device = None
     if a.serve_seconds > 0 and a.serve_target_ms > 0:
@@ -258,6 +272,7 @@ def main():
         "clipmap_update": clipmap,
         "batched": batched,
         "ray_queries": queries,
+        "mixed_batched_vs_plain": mix,
         "serving": serving,
         "serving_without_latency_target": serving_untargeted,
         "serving_device_builds": serving_device,
@@ -389,6 +404,104 @@ def batching(vx, scenes, sc, hw, ctx, n_views, steps):
             "plain_rays_per_s": round(rays / t_plain, 1), "batched_rays_per_s": round(rays / t_batch, 1),
             "speedup": round(t_plain / t_batch, 3), "plain_ms_per_step": round(t_plain * 1e3, 2),
             "batched_ms_per_step": round(t_batch * 1e3, 2), "tuned": tuned}
+
+
+def mixed(vx, scenes, n_views=64, n_batches=256, steps=2):
+    """The mixed workload of the target: the batching leg's 64 views (the same
+    seeded resolutions from {320x180 .. 3840x2160}) and the ray-query leg's 256
+    batches (the same seeded sizes, log-uniform 1e3..1e6 rays), over one world,
+    the 32x4x32-chunk seed-42 world with views along seeded flights.  Plain: one
+    launch sequence per view (vx_render) and one query launch per ray batch,
+    all on the context stream.  Batched: the views as one fused batch
+    (vx_batch_render, autotuned) on the context stream while all ray batches
+    run as one fused query launch (autotuned) on a second stream."""
+    rng_v = np.random.default_rng(7)
+    sizes_v = [(320, 180), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
+    res = [sizes_v[i] for i in rng_v.integers(0, len(sizes_v), n_views)]
+    rng_q = np.random.default_rng(11)
+    sizes_q = np.exp(rng_q.uniform(np.log(1e3), np.log(1e6), n_batches)).astype(np.int64)
+    n_rays = int(sizes_q.sum())
+    sc = scenes.c4(streams=n_views, frames=200)
+    hw = vx.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
+    clip, heights = scenes.scene_clipmap(sc, 0)
+    ctx = vx.Context(node_capacity=len(hw.nodes), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
+                     clip_levels=5, clip_res=256)
+    ctx.upload_world(hw)
+    ctx.write_clipmap(clip, heights)
+    qcam = vx.camera_build(vx.make_pose((1.0, 1.0, 1.0), 0.0, 0.0), (64, 64), (64, 64), 0, grid_origin=sc.grid_origin,
+                           grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
+    rays = scenes.query_rays(rng_q, n_rays, sc.grid_origin, sc.grid_dims, tmax_m=51.2)
+    d_rays = vx.DeviceArray.from_numpy(rays)
+    d_hits = vx.DeviceArray(n_rays * vx.HIT_DTYPE.itemsize)
+    offs = np.concatenate([[0], np.cumsum(sizes_q)])
+    stride = vx.HIT_DTYPE.itemsize
+    view_rays = int(sum(w * h for w, h in res))
+
+    def cams(step):
+        out = []
+        for i, (w, h) in enumerate(res):
+            pose, t_s = sc.stream_poses[i][step % len(sc.stream_poses[i])]
+            out.append(vx.camera_build(pose, (w, h), (w, h), step, time_s=t_s, grid_origin=sc.grid_origin,
+                                       grid_dims=sc.grid_dims, chunk_count=hw.chunk_count, jitter=True))
+        return out
+
+    def run(step_fn):
+        times = []
+        for step in range(steps + 1):
+            cs = cams(step)
+            ctx.sync()
+            t = time.perf_counter()
+            step_fn(cs)
+            if step > 0:  # step 0 warms up
+                times.append(time.perf_counter() - t)
+        return float(np.mean(times))
+
+    views = [vx.View(ctx, r, r) for r in res]
+
+    def plain_step(cs):
+        for v, c in zip(views, cs):
+            v.write_camera(c)
+            v.render(vx.PASS_FRAME)
+        for i in range(n_batches):
+            k = int(sizes_q[i])
+            vx.query_rays_device(ctx, qcam, d_rays.view(int(offs[i]) * 32, k * 32),
+                                 d_hits.view(int(offs[i]) * stride, k * stride), k)
+        ctx.sync()
+
+    t_plain = run(plain_step)
+    for v in views:
+        v.close()
+    views = [vx.View(ctx, r, r) for r in res]
+    batch = vx.Batch(ctx, views)
+    for v, c in zip(views, cams(0)):
+        v.write_camera(c)
+    tuned_views = batch.tune(vx.PASS_FRAME, iters=1).as_dict()
+    tuned_rays = vx.query_tune(ctx, qcam, d_rays, d_hits, n_rays).as_dict()
+    qs = vx.HipStream()
+
+    def batched_step(cs):
+        for v, c in zip(views, cs):
+            v.write_camera(c)
+        batch.render(vx.PASS_FRAME)
+        vx.query_rays_device(ctx, qcam, d_rays, d_hits, n_rays, stream=qs.handle.value)
+        ctx.sync()
+        qs.sync()
+
+    t_batch = run(batched_step)
+    batch.close()
+    for v in views:
+        v.close()
+    qs.close()
+    d_rays.close()
+    d_hits.close()
+    ctx.close()
+    total = view_rays + n_rays
+    return {"workload": f"{n_views} views (320x180..3840x2160, seed 7) + {n_batches} ray batches (1e3..1e6 rays, "
+                        f"seed 11) per step, 32x4x32-chunk world + 5-level clipmap, full pipeline for the views",
+            "view_rays": view_rays, "query_rays": n_rays,
+            "plain_rays_per_s": round(total / t_plain, 1), "batched_rays_per_s": round(total / t_batch, 1),
+            "speedup": round(t_plain / t_batch, 3), "plain_ms_per_step": round(t_plain * 1e3, 2),
+            "batched_ms_per_step": round(t_batch * 1e3, 2), "tuned_views": tuned_views, "tuned_rays": tuned_rays}
 
 
 def ray_queries(vx, scenes, n_batches):

@@ -631,6 +631,35 @@ class View:
             pass
 
 
+class HipStream:
+    """A non-blocking HIP stream at the default priority (for passing to the C
+    ABI calls that take a stream)."""
+
+    def __init__(self):
+        rt = DeviceArray._rt()
+        rt.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+        rt.hipStreamSynchronize.argtypes = [C.c_void_p]
+        rt.hipStreamDestroy.argtypes = [C.c_void_p]
+        self.handle = C.c_void_p()
+        if rt.hipStreamCreateWithFlags(C.byref(self.handle), 1) != 0:  # hipStreamNonBlocking
+            raise VoxhipError("hipStreamCreateWithFlags failed")
+
+    def sync(self) -> None:
+        if DeviceArray._rt().hipStreamSynchronize(self.handle) != 0:
+            raise VoxhipError("hipStreamSynchronize failed")
+
+    def close(self) -> None:
+        if self.handle:
+            DeviceArray._rt().hipStreamDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class ChunkBuilder:
     """The chunk builder on the device (vx_chunk_builder_*): count a batch of
     chunks whose material (n x 64^3 u8, y-slab) and ground (n x 64^2 i32) are
