@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -270,6 +271,48 @@ int vxd_cpu_convert(const float* x, uint32_t n, int32_t* out_i, uint32_t* out_u)
   }
   return 0;
 }
+
+// The device's constant-divisor quotient (divc<k>) against the correctly
+// rounded division, over the non-negative binary32 bit patterns [lo, hi):
+// returns the number of inputs where they differ (NaN results compare equal).
+// Threads split the range.  Hardware fma for the two fmaf of divc.
+extern "C++" {
+template <uint32_t K>
+__attribute__((target("fma"))) static uint64_t fastdiv_range(uint32_t lo, uint32_t hi) {
+  constexpr float d = FAST_DIVISORS[K];
+  uint64_t bad = 0;
+  for (uint32_t b = lo; b < hi; ++b) {
+    float x;
+    std::memcpy(&x, &b, 4);
+    const float f = divc<K>(x), r = x / d;
+    uint32_t fb, rb;
+    std::memcpy(&fb, &f, 4);
+    std::memcpy(&rb, &r, 4);
+    bad += (fb != rb) && !(f != f && r != r);
+  }
+  return bad;
+}
+}  // extern "C++"
+uint64_t vxd_cpu_fastdiv_mismatches(uint32_t k, uint32_t lo, uint32_t hi, uint32_t threads) {
+  static uint64_t (*const fns[])(uint32_t, uint32_t) = {fastdiv_range<0>, fastdiv_range<1>, fastdiv_range<2>,
+                                                         fastdiv_range<3>, fastdiv_range<4>, fastdiv_range<5>};
+  static_assert(sizeof(fns) / sizeof(fns[0]) == DIV_COUNT, "one checker per divisor");
+  if (k >= DIV_COUNT || hi < lo) return ~0ull;
+  threads = std::max(1u, threads);
+  std::vector<uint64_t> bad(threads, 0);
+  std::vector<std::thread> pool;
+  const uint64_t span = (uint64_t)hi - lo, per = (span + threads - 1) / threads;
+  for (uint32_t t = 0; t < threads; ++t) {
+    const uint64_t a = lo + std::min<uint64_t>(span, t * per), b = lo + std::min<uint64_t>(span, (t + 1) * per);
+    pool.emplace_back([&, t, a, b] { bad[t] = fns[k]((uint32_t)a, (uint32_t)b); });
+  }
+  for (auto& th : pool) th.join();
+  uint64_t sum = 0;
+  for (uint64_t v : bad) sum += v;
+  return sum;
+}
+float vxd_cpu_fast_divisor(uint32_t k) { return k < DIV_COUNT ? FAST_DIVISORS[k] : 0.0f; }
+uint32_t vxd_cpu_fast_divisor_count() { return DIV_COUNT; }
 
 // The device value noise and fbm, elementwise: out[2n] = value_noise, fbm at (x, y).
 int vxd_cpu_noise(const float* x, const float* y, uint32_t n, float* out) {

@@ -146,3 +146,30 @@ def test_device_value_noise_matches_oracle():
     want = np.array([[L.vxo_value_noise(float(a), float(b)), L.vxo_fbm(float(a), float(b))] for a, b in xy], np.float32)
     same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
     assert same.all(), xy[np.argwhere(~same)[:5, 0]].tolist()
+
+
+def test_constant_divisor_quotients_are_correctly_rounded():
+    """Every entry of the device's constant-divisor table: the reciprocal
+    multiply plus fma correction (divc) equals the correctly rounded division
+    for all 2^31 non-negative binary32 inputs (zero, subnormals, normals,
+    infinity, NaN); the method is odd in x, so this covers every input."""
+    import vxd_cpu
+
+    ds = vxd_cpu.fast_divisors()
+    assert len(ds) == 6 and all(d > 0.0 for d in ds)
+    for k in range(len(ds)):
+        assert vxd_cpu.fastdiv_mismatches(k) == 0, (k, ds[k])
+
+
+def test_constant_divisor_check_catches_a_wrong_method():
+    """The exhaustive check is not vacuous: a plain reciprocal multiply (no
+    correction) is wrong for some inputs of every divisor that is not a power
+    of two, which the same comparison finds on a small range."""
+    import numpy as np
+    import vxd_cpu
+
+    for d in vxd_cpu.fast_divisors():
+        x = np.arange(0x3F800000, 0x3F800000 + (1 << 20), dtype=np.uint32).view(np.float32)
+        plain = (x * np.float32(np.float32(1.0) / np.float32(d))).astype(np.float32)
+        exact = (x / np.float32(d)).astype(np.float32)
+        assert np.any(plain != exact), d

@@ -107,3 +107,19 @@ def noise(x: np.ndarray, y: np.ndarray) -> np.ndarray:
     out = np.zeros((len(x), 2), np.float32)
     assert lib().vxd_cpu_noise(x.ctypes.data, y.ctypes.data, len(x), out.ctypes.data) == 0
     return out
+
+
+def fastdiv_mismatches(k: int, lo: int = 0, hi: int = 0x80000000, threads: int = 8) -> int:
+    """Inputs in the non-negative bit-pattern range [lo, hi) where the device's
+    divc<k> differs from the correctly rounded division."""
+    f = lib().vxd_cpu_fastdiv_mismatches
+    f.restype = C.c_uint64
+    f.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+    return int(f(k, lo, hi, threads))
+
+
+def fast_divisors() -> list:
+    lib().vxd_cpu_fast_divisor.restype = C.c_float
+    lib().vxd_cpu_fast_divisor.argtypes = [C.c_uint32]
+    lib().vxd_cpu_fast_divisor_count.restype = C.c_uint32
+    return [float(lib().vxd_cpu_fast_divisor(k)) for k in range(lib().vxd_cpu_fast_divisor_count())]

@@ -69,4 +69,46 @@ constexpr float ALBEDO_VAR_GAIN = 3.5f;
 constexpr uint32_t MACRO_DIM = 8, TILE_SIZE = 8, MAX_TILE_CHUNKS = 192, PRIMARY_MAX_TILE_CHUNKS = 24;
 constexpr float U32_TO_F01 = 1.0f / 4294967296.0f;
 
+// Constant divisors whose quotients the passes compute with a two-term
+// reciprocal (divc below) instead of the full division sequence.  For every
+// entry divc equals the correctly rounded x / d for every binary32 x: checked
+// exhaustively over all non-negative bit patterns (divc is odd in x), so the
+// quotients stay bit-identical to a division.  Entries equal the divisor each
+// call site's expression yields.
+constexpr float FAST_DIVISORS[] = {
+    GODRAY_EDGE1 - GODRAY_EDGE0,                         // godray shaft edge smoothstep
+    0.80f - 0.35f,                                       // godray sun gate smoothstep
+    GODRAY_HAZE_NEAR_FADE,                               // godray haze ramp
+    CLOUD_TOP_H - CLOUD_BASE_H,                          // cloud height fraction
+    1.00f - 0.78f,                                       // cloud height upper smoothstep
+    (CLOUD_COVERAGE + CLOUD_SOFTNESS) - CLOUD_COVERAGE,  // cloud coverage smoothstep
+};
+enum : uint32_t { DIV_GODRAY_EDGE, DIV_GODRAY_GATE, DIV_HAZE_FADE, DIV_CLOUD_H, DIV_CLOUD_HI, DIV_COVERAGE,
+                  DIV_COUNT };
+static_assert(sizeof(FAST_DIVISORS) / sizeof(float) == DIV_COUNT, "one divisor per name");
+
+// x / FAST_DIVISORS[K] as fma(x, zh, x * zl), where zh + zl is 1/d to about
+// twice binary32 precision: zh = 1/d rounded, zl = (1/d - zh) rounded
+// (Brisebarre, Muller, Raina, "Accelerating correctly rounded floating-point
+// division when the divisor is known in advance", IEEE TC 53(8), 2004).  For
+// |x| below 2^-100 x * zl loses bits (and the method has no guarantee), so
+// those inputs, zero, infinities and NaN take the division itself.  Some
+// divisors have a few failing x even in range (0.12 has 225); such divisors are
+// not in the table.
+template <uint32_t K>
+VXD float divc(float x) {
+  constexpr float d = FAST_DIVISORS[K];
+  constexpr float zh = 1.0f / d;
+  constexpr float zl = (float)(1.0 / (double)d - (double)zh);
+  const float ax = fabsf(x);
+  if (__builtin_expect(ax >= 0x1p-100f && ax <= 0x1p100f, 1)) return fmaf(x, zh, x * zl);
+  return x / d;
+}
+// smoothstepf(lo, lo + FAST_DIVISORS[K], x) for a call site whose hi - lo is entry K.
+template <uint32_t K>
+VXD float smoothstepc(float lo, float x) {
+  float t = clampf(divc<K>(x - lo), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+
 }  // namespace vxd
