@@ -624,21 +624,25 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
         else:
             k = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
             off = int(rng.integers(0, len(pool) - k))
-            ok = srv.submit_rays(qcam, pool[off:off + k], tag=i)
+            ok = srv.submit_rays(qcam, pool[off:off + k], tag=i, out=np.empty(k, vx.HIT_DTYPE))
             if ok:
                 ray_rays += k
         submitted += 1
         rejected += 0 if ok else 1
+        while srv.poll() is not None:  # results release their hit buffers
+            pass
     stop.set()
     editor.join()
     srv.drain()
     wall = time.perf_counter() - t0
     cpu_s = time.process_time() - cpu0
+    while srv.poll() is not None:
+        pass
     srv.wait_edits()
     st = srv.stats()
     srv.close()
     return {"workload": f"Poisson {rate:g} req/s for {seconds:g} s (seed 5), half frames of {n_streams} view streams "
-                        f"(320x180..3840x2160, at most 1 pending frame each), half ray batches (1e3..1e6 rays), "
+                        f"(320x180..3840x2160, at most 1 pending frame each), half ray batches (1e3..1e6 rays, hits returned), "
                         f"{edit_rate:g} single-voxel edits/s, 32x4x32 chunks + 5-level clipmap, {budget >> 30} GB budget, "
                         + (f"{target_ms:g} ms latency target" if target_ms > 0 else "no latency target")
                         + (", edited chunks rebuilt on the device" if device_build else

@@ -441,11 +441,14 @@ uint64_t request_rays(const Pending& p) {
 
 void device_main(vx_server* s) {
   hipSetDevice(vx_internal_device(s->ctx));
-  std::vector<vx_ray> all_rays;
+  // Rays go up and hits come down through pinned staging, so the copies run on
+  // the DMA engines instead of as staging blits on the compute units.
   vx_ray* d_rays = nullptr;
   vx_hit* d_hits = nullptr;
   size_t d_cap = 0;
-  std::vector<vx_hit> h_hits;
+  vx_ray* h_rays = nullptr;
+  vx_hit* h_hits = nullptr;
+  size_t h_cap = 0;
   for (;;) {
     std::vector<Pending> views, rays;
     std::vector<ChunkBuild> builds;
@@ -547,9 +550,25 @@ void device_main(vx_server* s) {
     size_t nr = 0;
     for (Pending& p : rays) nr += p.rays.size();
     if (!rc && nr) {
-      all_rays.clear();
-      for (Pending& p : rays) all_rays.insert(all_rays.end(), p.rays.begin(), p.rays.end());
-      if (nr > d_cap) {
+      if (nr > h_cap) {
+        if (h_rays) hipHostFree(h_rays);
+        if (h_hits) hipHostFree(h_hits);
+        h_rays = nullptr;
+        h_hits = nullptr;
+        h_cap = 0;
+        const size_t cap = std::max(nr, (size_t)1 << 20);
+        if (hipHostMalloc((void**)&h_rays, cap * sizeof(vx_ray), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&h_hits, cap * sizeof(vx_hit), hipHostMallocDefault) != hipSuccess)
+          rc = vx_fail(VX_E_NOMEM, "server: pinned ray staging");
+        else
+          h_cap = cap;
+      }
+      size_t at = 0;
+      for (Pending& p : rays) {
+        if (!rc && !p.rays.empty()) std::memcpy(h_rays + at, p.rays.data(), p.rays.size() * sizeof(vx_ray));
+        at += p.rays.size();
+      }
+      if (!rc && nr > d_cap) {
         if (d_rays) hipFree(d_rays);
         if (d_hits) hipFree(d_hits);
         d_rays = nullptr;
@@ -561,17 +580,22 @@ void device_main(vx_server* s) {
         else
           d_cap = nr;
       }
-      if (!rc && hipMemcpyAsync(d_rays, all_rays.data(), nr * sizeof(vx_ray), hipMemcpyHostToDevice, s->qstream) != hipSuccess)
+      if (!rc && hipMemcpyAsync(d_rays, h_rays, nr * sizeof(vx_ray), hipMemcpyHostToDevice, s->qstream) != hipSuccess)
         rc = vx_fail(VX_E_DEVICE, "server: ray upload");
       if (!rc) rc = vx_query_rays(s->ctx, &rays[0].req.camera, d_rays, (uint32_t)nr, d_hits, s->qstream);
+      // Hit downloads queue behind the query on its stream (only the ranges of
+      // requests that asked for their hits), overlapping the view passes.
+      size_t off = 0;
+      for (Pending& p : rays) {
+        if (!rc && p.req.out_hits && !p.rays.empty() &&
+            hipMemcpyAsync(h_hits + off, d_hits + off, p.rays.size() * sizeof(vx_hit), hipMemcpyDeviceToHost,
+                           s->qstream) != hipSuccess)
+          rc = vx_fail(VX_E_DEVICE, "server: hit download");
+        off += p.rays.size();
+      }
     }
     if (!rc && batch) rc = vx_sync(s->ctx);
-    if (!rc && nr) {
-      h_hits.resize(nr);
-      if (hipMemcpyAsync(h_hits.data(), d_hits, nr * sizeof(vx_hit), hipMemcpyDeviceToHost, s->qstream) != hipSuccess ||
-          hipStreamSynchronize(s->qstream) != hipSuccess)
-        rc = vx_fail(VX_E_DEVICE, "server: hit download");
-    }
+    if (!rc && nr && hipStreamSynchronize(s->qstream) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "server: query stream");
     // Outputs and results.
     for (size_t i = 0; i < views.size() && !rc; ++i)
       if (views[i].req.out_rgba)
@@ -579,7 +603,7 @@ void device_main(vx_server* s) {
                             (size_t)views[i].req.width * views[i].req.height * 16);
     size_t at = 0;
     for (Pending& p : rays) {
-      if (!rc && p.req.out_hits) std::memcpy(p.req.out_hits, h_hits.data() + at, p.rays.size() * sizeof(vx_hit));
+      if (!rc && p.req.out_hits) std::memcpy(p.req.out_hits, h_hits + at, p.rays.size() * sizeof(vx_hit));
       at += p.rays.size();
     }
     if (batch) vx_batch_destroy(batch);
@@ -623,6 +647,8 @@ void device_main(vx_server* s) {
   }
   if (d_rays) hipFree(d_rays);
   if (d_hits) hipFree(d_hits);
+  if (h_rays) hipHostFree(h_rays);
+  if (h_hits) hipHostFree(h_hits);
 }
 
 }  // namespace
