@@ -607,6 +607,7 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
     editor.start()
     submitted = rejected = 0
     view_rays = ray_rays = 0
+    done = []
     for i, ta in enumerate(t_arr):
         dt = t0 + ta - time.perf_counter()
         if dt > 0:
@@ -629,15 +630,18 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
                 ray_rays += k
         submitted += 1
         rejected += 0 if ok else 1
-        while srv.poll() is not None:  # results release their hit buffers
-            pass
+        while (r := srv.poll()) is not None:  # results release their hit buffers
+            done.append((r.submit_ms, r.start_ms, r.done_ms))
     stop.set()
     editor.join()
     srv.drain()
     wall = time.perf_counter() - t0
     cpu_s = time.process_time() - cpu0
-    while srv.poll() is not None:
-        pass
+    while (r := srv.poll()) is not None:
+        done.append((r.submit_ms, r.start_ms, r.done_ms))
+    d = np.array(done) if done else np.zeros((1, 3))
+    lat = d[:, 2] - d[:, 0]
+    slow = d[lat >= np.percentile(lat, 99)]
     srv.wait_edits()
     st = srv.stats()
     srv.close()
@@ -653,7 +657,11 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
             "view_latency_ms": {"p50": round(st["view_p50_ms"], 2), "p99": round(st["view_p99_ms"], 2)},
             "rays_latency_ms": {"p50": round(st["rays_p50_ms"], 2), "p99": round(st["rays_p99_ms"], 2)},
             "edits_applied_chunks": int(st["edits_applied"]), "gpu_busy_frac": round(st["busy_ms"] / (wall * 1e3), 3),
-            "host_cpu_s": round(cpu_s, 2), "wall_s": round(wall, 2)}
+            "host_cpu_s": round(cpu_s, 2), "wall_s": round(wall, 2),
+            "slowest_1pct_ms": {"wait_for_launch": round(float(np.mean(slow[:, 1] - slow[:, 0])), 2),
+                                "in_launch": round(float(np.mean(slow[:, 2] - slow[:, 1])), 2),
+                                "launch_ms_p50": round(float(np.percentile(d[:, 2] - d[:, 1], 50)), 2),
+                                "launch_ms_max": round(float(np.max(d[:, 2] - d[:, 1])), 2)}}
 
 
 def chunk_builds(vx, scenes, repeats=3, cpu_sample=24):
