@@ -206,25 +206,50 @@ def main():
 
     roofline = None
     counts = None
+    passes = None
     if not a.no_count:
-        # The counting form of the pass (the reference algorithm's loads per ray), over the first three timed frames.
+        # The counting forms of the passes (the reference algorithm's loads per
+        # ray), over the first three timed frames: primary, then godray.
         view.reset()
         tot = {n: 0 for n in vx.COUNT_NAMES}
+        tot_g = {n: 0 for n in vx.COUNT_NAMES}
         ncount = min(3, a.steps)
+        ray_b = []
         view.counters(reset=True)
         for k in range(ncount):
             view.write_camera(cams[a.warmup + k])
-            view.render(vx.PASS_PRIMARY | vx.PASS_COUNT | vx.PASS_END_FRAME)
+            view.render(vx.PASS_PRIMARY | vx.PASS_COUNT)
             c = view.counters(reset=True)
+            ray_b.append(view.read(vx.OUT_RAY_BYTES).ravel())
+            view.render(vx.PASS_GODRAY | vx.PASS_COUNT | vx.PASS_END_FRAME)
+            cg = view.counters(reset=True)
             for n in tot:
                 tot[n] += c[n]
+                tot_g[n] += cg[n]
         counts = {n: v / ncount for n, v in tot.items()}
         bytes_per_launch = vx.algorithmic_bytes(counts)
+        rb = np.concatenate(ray_b).astype(np.float64)
         achieved = bytes_per_launch / (prim_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(W, H),
                     "kernel": "k_primary_batch", "algorithmic_bytes_per_launch": int(bytes_per_launch),
-                    "algorithmic_bytes_per_ray": round(bytes_per_launch / rays, 1), "profiled": profiled_valu()}
+                    "algorithmic_bytes_per_ray": {"mean": round(bytes_per_launch / rays, 1),
+                                                  "p50": round(float(np.percentile(rb, 50)), 1),
+                                                  "p99": round(float(np.percentile(rb, 99)), 1),
+                                                  "max": round(float(rb.max()), 1),
+                                                  "sum_check": round(float(rb.sum()) / ncount / max(bytes_per_launch, 1), 6)},
+                    "profiled": profiled_valu()}
+        # The other passes against the HBM roof: godray's counted loads, and the
+        # streaming passes' fixed bytes per pixel (local TAA 48, composite 212,
+        # composite TAA 64).
+        godray_b = vx.algorithmic_bytes({n: v / ncount for n, v in tot_g.items()})
+        pm = pass_ms.mean(axis=0)
+        passes = {}
+        for name, bts, ms in (("godray", godray_b, pm[2]), ("local_taa", 48 * rays, pm[1]),
+                              ("composite", 212 * rays, pm[3]), ("composite_taa", 64 * rays, pm[4])):
+            gbs = bts / (ms * 1e-3) / 1e9
+            passes[name] = {"bytes_per_launch": int(bts), "ms": round(float(ms), 4), "achieved_gbs": round(gbs, 1),
+                            "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
     lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup,
                                                     not a.static_clipmap)
@@ -267,6 +292,7 @@ def main():
                          zip(("primary", "local_taa", "godray", "composite", "composite_taa"), pass_ms.mean(axis=0))},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "roofline": roofline,
+        "pass_roofline_hbm": passes,
         "cpu_baseline": cpu,
         "latency_1080p_ms": lat1080,
         "clipmap_update": clipmap,

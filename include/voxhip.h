@@ -257,6 +257,10 @@ int vx_render(vx_view* view, uint32_t pass_mask, void* stream);
 #define VX_OUT_PRE_TAA        7  /* present res RGBA32F (composite output)     */
 #define VX_OUT_OUTPUT         8  /* present res RGBA32F (composite TAA output) */
 #define VX_OUT_HITS           9  /* render res vx_hit                          */
+/* render res R32F: per ray, the bytes of the reference algorithm's loads and
+ * stores counted by the last VX_PASS_COUNT primary pass (the tile's candidate
+ * gather shared over its 64 rays); no reference counterpart. */
+#define VX_OUT_RAY_BYTES     10
 /* Synchronous device-to-host copy of one output (waits for the view's stream). */
 int vx_read_output(vx_view* view, int which, void* dst, size_t bytes);
 /* Device pointer of one output, for zero-copy consumers on the same device. */

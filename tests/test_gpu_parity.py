@@ -205,3 +205,42 @@ def test_flight_c1_world(gpu, staged):
         _report(f"c1 flight frame {frame} staged={staged}", s)
         _assert_parity(s)
         assert s["output_max_abs"] <= COLOR_ABS_TOL
+
+
+def test_counted_loads_match_oracle(gpu):
+    """The counting form of the primary and godray passes (the roofline's
+    algorithmic bytes) against the oracle's own load counts for the same frame:
+    every counter equal; the per-ray byte totals sum to the launch's bytes."""
+    O = _oracle()
+    from voxhip import scenes
+
+    sc, hw = host_world("c0")
+    clip, heights = scenes.scene_clipmap(sc)
+    cam = scenes.frame_camera(sc, hw, 0, wind_strength=2.0)
+    ctx = make_context(gpu, hw)
+    ctx.write_clipmap(clip, heights)
+    view = gpu.View(ctx, sc.present, sc.render)
+    with pytest.raises(gpu.VoxhipError, match="counting"):
+        view.read(gpu.OUT_RAY_BYTES)
+    view.write_camera(cam)
+    view.counters(reset=True)
+    view.render(gpu.PASS_PRIMARY | gpu.PASS_COUNT)
+    c_prim = view.counters(reset=True)
+    ray_bytes = view.read(gpu.OUT_RAY_BYTES).astype(np.float64)
+    view.render(gpu.PASS_GODRAY | gpu.PASS_COUNT | gpu.PASS_END_FRAME)
+    c_god = view.counters(reset=True)
+
+    w = O.World.from_host(hw, clip, heights)
+    fr = O.Frame(*sc.render)
+    oc = O.Counts()
+    O.primary(w, cam, fr, gpu.HIT_DTYPE, counts=oc)
+    og = O.Counts()
+    O.godray(w, cam, fr, counts=og)
+    _report("counted loads", {"device_primary": c_prim, "oracle_primary": oc.as_dict(), "device_godray": c_god,
+                              "oracle_godray": og.as_dict()})
+    for n in gpu.COUNT_NAMES:
+        assert c_prim[n] == oc.as_dict()[n], n
+        assert c_god[n] == og.as_dict()[n], n
+    # 640x360 is a whole number of 8x8 tiles: the tile-gather shares add up exactly
+    assert ray_bytes.sum() == gpu.algorithmic_bytes(c_prim)
+    assert ray_bytes.min() > 0

@@ -42,6 +42,14 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t ntiles) {
   return x * per + min(x, rem) + local;
 }
 
+// Bytes of a lane's counted loads and stores: element counts times the element
+// sizes (grid word, ChunkMetaGpu, macro word, NodeGpu, NodeRopesGpu, colinfo
+// word, clip texel; texture traffic is counted in bytes).
+__device__ __forceinline__ uint32_t count_bytes(const Counts& c) {
+  return 4u * c.grid + 48u * c.meta + 4u * c.macro + 16u * c.node + 32u * c.rope + 4u * c.colinfo + 4u * c.clip +
+         c.tex_rd + c.tex_wr;
+}
+
 template <bool COUNT>
 __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long* out) {
   if (!COUNT) return;
@@ -72,7 +80,7 @@ __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long
 template <bool COUNT>
 __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary(
     Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, uint32_t tiles_x, uint32_t ntiles,
-    unsigned long long* __restrict__ counters) {
+    unsigned long long* __restrict__ counters, float* __restrict__ ray_bytes) {
   __shared__ TileLDS L;
   const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
   const uint32_t wgx = tile % tiles_x, wgy = tile / tiles_x;
@@ -87,8 +95,12 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary(
     Counts cnt = {};
     C.cnt = &cnt;
     if (threadIdx.x == 0) counted::tile_build(C, L, wgx, wgy);
+    // Lane 0's counts so far are the tile's candidate gather, shared by its 64 rays.
+    const uint32_t tile_b = threadIdx.x == 0 ? count_bytes(cnt) : 0u;
     __syncthreads();
     if (gx < fb.rw && gy < fb.rh) counted::primary_pixel(C, L, fb, gx, gy, counted::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+    const float share = (float)__shfl(tile_b, 0, 64) * (1.0f / 64.0f);
+    if (ray_bytes && gx < fb.rw && gy < fb.rh) ray_bytes[(size_t)gy * fb.rw + gx] = (float)(count_bytes(cnt) - tile_b) + share;
     flush_counts<true>(cnt, counters);
   } else {
     plain::Ctx C;
@@ -524,10 +536,10 @@ hipError_t vx_launch_primary_staged(const vxd::Scene& S, const vx_camera* cam_de
   return hipGetLastError();
 }
 hipError_t vx_launch_primary(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
-                             unsigned long long* counters, hipStream_t st) {
+                             unsigned long long* counters, float* ray_bytes, hipStream_t st) {
   const uint32_t tx = (fb.rw + 7u) / 8u, ty = (fb.rh + 7u) / 8u, nt = tx * ty;
-  if (count) hipLaunchKernelGGL(k_primary<true>, dim3(nt), dim3(64), 0, st, S, cam_dev, fb, tx, nt, counters);
-  else hipLaunchKernelGGL(k_primary<false>, dim3(nt), dim3(64), 0, st, S, cam_dev, fb, tx, nt, counters);
+  if (count) hipLaunchKernelGGL(k_primary<true>, dim3(nt), dim3(64), 0, st, S, cam_dev, fb, tx, nt, counters, ray_bytes);
+  else hipLaunchKernelGGL(k_primary<false>, dim3(nt), dim3(64), 0, st, S, cam_dev, fb, tx, nt, counters, nullptr);
   return hipGetLastError();
 }
 hipError_t vx_launch_local_taa(const vxd::FrameBuffers& fb, hipStream_t st) {

@@ -5,9 +5,10 @@
 
 #include "vxd_scene.h"
 
-// The primary pass as one kernel per 8x8 tile (count: the load-counting form).
+// The primary pass as one kernel per 8x8 tile (count: the load-counting form,
+// which also writes each ray's counted bytes to ray_bytes when it is not null).
 hipError_t vx_launch_primary(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
-                             unsigned long long* counters, hipStream_t st);
+                             unsigned long long* counters, float* ray_bytes, hipStream_t st);
 // The primary pass as staged kernels (trace, grass refinement, shadow, voxel
 // shading, light gather, heightfield/sky) over pixel queues.
 hipError_t vx_launch_primary_staged(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb,

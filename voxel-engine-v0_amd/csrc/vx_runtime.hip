@@ -105,6 +105,7 @@ struct vx_view {
   uint32_t* prim_q[vxd::kNumPrimQueues] = {};
   uint32_t* prim_count = nullptr;
   unsigned long long* counters = nullptr;
+  float* ray_bytes = nullptr;  // per ray: counted bytes of the last counting primary pass (allocated on first use)
   // Slot written last by each history-producing pass.
   int written_primary = 1, written_local = 1, written_godray = 1, written_out = 1;
   hipEvent_t ev[10] = {};
@@ -198,6 +199,7 @@ void free_view_buffers(vx_view* v) {
   dfree(v->prim_count);
   dfree(v->ws_tab);
   dfree(v->ws_buf);
+  dfree(v->ray_bytes);
 }
 
 vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits);
@@ -622,7 +624,11 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
   if (!count && (mask & VX_PASS_PRIMARY) && !(c->flags & VX_CTX_PRIMARY_STAGED)) VX_HIP(vx_launch_ws_begin(B1, st));
   if (mask & VX_PASS_PRIMARY) {
     if (count) {
-      VX_HIP(vx_launch_primary(S, v->cam_dev, fb, true, v->counters, st));
+      if (!v->ray_bytes) {
+        const int rc = dalloc(&v->ray_bytes, (size_t)v->rw * v->rh);
+        if (rc) return rc;
+      }
+      VX_HIP(vx_launch_primary(S, v->cam_dev, fb, true, v->counters, v->ray_bytes, st));
     } else if (!(c->flags & VX_CTX_PRIMARY_STAGED)) {
       VX_HIP(vx_launch_primary_ws(S, B1, persistent, st));
     } else {
@@ -670,6 +676,11 @@ static int output_buffer(vx_view* v, int which, void** p, size_t* bytes) {
     case VX_OUT_PRE_TAA: *p = v->pre_taa; *bytes = pp * 16; return VX_OK;
     case VX_OUT_OUTPUT: *p = v->output; *bytes = pp * 16; return VX_OK;
     case VX_OUT_HITS: *p = v->hits; *bytes = r * sizeof(vx_hit); return VX_OK;
+    case VX_OUT_RAY_BYTES:
+      if (!v->ray_bytes) return vx_fail(VX_E_INVALID, "VX_OUT_RAY_BYTES: no counting primary pass has run on this view");
+      *p = v->ray_bytes;
+      *bytes = r * 4;
+      return VX_OK;
     default: return vx_fail(VX_E_INVALID, "bad output selector %d", which);
   }
 }

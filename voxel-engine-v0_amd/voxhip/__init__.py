@@ -43,6 +43,7 @@ PASS_COUNT = 0x200
 
 OUT_COLOR, OUT_DEPTH, OUT_LOCAL, OUT_PRIMARY_HIST, OUT_SHADOW_HIST = 0, 1, 2, 3, 4
 OUT_LOCAL_HIST, OUT_GODRAY, OUT_PRE_TAA, OUT_OUTPUT, OUT_HITS = 5, 6, 7, 8, 9
+OUT_RAY_BYTES = 10
 
 ARENA_NODES, ARENA_ROPES, ARENA_META, ARENA_MACRO, ARENA_COLINFO = range(5)
 CTX_GODRAY_MACRO_FIX = 1
@@ -598,6 +599,7 @@ class View:
             OUT_SHADOW_HIST: ((self.rh, self.rw), np.float32), OUT_LOCAL_HIST: ((self.rh, self.rw, 4), np.float32),
             OUT_GODRAY: ((self.rh, self.rw, 4), np.float32), OUT_PRE_TAA: ((self.ph, self.pw, 4), np.float32),
             OUT_OUTPUT: ((self.ph, self.pw, 4), np.float32), OUT_HITS: ((self.rh, self.rw), HIT_DTYPE),
+            OUT_RAY_BYTES: ((self.rh, self.rw), np.float32),
         }
         shape, dt = shapes[which]
         out = np.empty(shape, dt)
