@@ -55,7 +55,8 @@ def parse():
                     help="render the 4K view through a one-view batch (work-stealing launches) instead of vx_render")
     ap.add_argument("--no-chunk-builds", action="store_true", help="skip the chunk builder measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed views + ray batches workload")
-    ap.add_argument("--only", choices=["mixed", "chunk_builds", "ray_queries", "serving", "serving_device_builds"],
+    ap.add_argument("--only", choices=["mixed", "chunk_builds", "ray_queries", "serving", "serving_device_builds",
+                                       "config0"],
                     help="run one workload alone and print its result (not the benchmark line)")
     ap.add_argument("--static-clipmap", action="store_true",
                     help="keep frame 0's clipmap instead of updating it every frame")
@@ -133,6 +134,7 @@ def main():
     if a.only:
         # One workload alone, its JSON object on one line (for profiling a leg).
         legs = {"mixed": lambda: mixed(vx, scenes), "chunk_builds": lambda: chunk_builds(vx, scenes),
+                "config0": lambda: config0(vx, scenes),
                 "ray_queries": lambda: ray_queries(vx, scenes, a.query_batches),
                 "serving": lambda: serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms),
                 "serving_device_builds": lambda: serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms,
@@ -267,7 +269,8 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(sc, hw, clip, heights, cams[a.warmup], W, H, a.cpu_rows)
+        cpu = cpu_baseline(sc, hw, clip, heights, cams[a.warmup], W, H, a.cpu_rows, frame=a.warmup)
+    c0 = None if a.no_cpu_baseline else config0(vx, scenes)
 
     out = {
         "metric": "primary rays/s at 4K + p50/p99 per-view latency, served batched vs plain (1 GPU)",
@@ -294,6 +297,7 @@ def main():
         "roofline": roofline,
         "pass_roofline_hbm": passes,
         "cpu_baseline": cpu,
+        "config0_primary_640x360": c0,
         "latency_1080p_ms": lat1080,
         "clipmap_update": clipmap,
         "batched": batched,
@@ -749,7 +753,51 @@ def chunk_builds(vx, scenes, repeats=3, cpu_sample=24):
             "host_sample_chunks": int(len(pick))}
 
 
-def cpu_baseline(sc, hw, clip, heights, cam, W, H, rows):
+def config0(vx, scenes, runs=7, cpu_runs=5):
+    """The reference's CPU-runnable case: the plain primary pass at 640x360 over
+    the 8x4x8-chunk seed-42 world, one camera at the fixed seeded pose, frame 1
+    (no history, the view reset before each run), 5-level clipmap.  GPU: median
+    of the primary pass's event time over `runs`; CPU: the oracle's main_primary
+    on the host cores, median of `cpu_runs` whole frames."""
+    sys.path.insert(0, str(REPO))
+    from oracle import oracle as O
+
+    sc = scenes.c0()
+    hw = vx.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
+    clip, heights = scenes.scene_clipmap(sc)
+    cam = scenes.frame_camera(sc, hw, 0)
+    ctx = vx.Context(node_capacity=len(hw.nodes), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
+                     clip_levels=5, clip_res=256)
+    ctx.upload_world(hw)
+    ctx.write_clipmap(clip, heights)
+    view = vx.View(ctx, sc.present, sc.render)
+    ms = []
+    for _ in range(runs + 1):
+        view.reset()
+        view.write_camera(cam)
+        view.render(vx.PASS_PRIMARY | vx.PASS_END_FRAME)
+        ms.append(view.timings()[0])
+    view.close()
+    ctx.close()
+    gpu_ms = float(np.median(ms[1:]))
+    threads = max(1, min(16, os.cpu_count() or 1))
+    w = O.World.from_host(hw, clip, heights)
+    cpu = []
+    for _ in range(cpu_runs):
+        fr = O.Frame(*sc.render)
+        t = time.perf_counter()
+        O.primary(w, cam, fr, threads=threads)
+        cpu.append(time.perf_counter() - t)
+    cpu_s = float(np.median(cpu))
+    rays = sc.render[0] * sc.render[1]
+    return {"workload": "plain primary pass, 640x360, 8x4x8-chunk seed-42 world, fixed seeded pose, frame 1, "
+                        "5-level clipmap",
+            "gpu_primary_ms_median": round(gpu_ms, 4), "gpu_rays_per_s": round(rays / (gpu_ms * 1e-3), 1),
+            "cpu_ms_median": round(cpu_s * 1e3, 2), "cpu_rays_per_s": round(rays / cpu_s, 1), "cpu_cores": threads,
+            "cpu_kind": "port", "gpu_over_cpu": round(cpu_s / (gpu_ms * 1e-3), 1)}
+
+
+def cpu_baseline(sc, hw, clip, heights, cam, W, H, rows, frame=0):
     """The oracle's main_primary on a band of tile rows through the middle of
     the same 4K frame (by default all of it), timed on the host cores."""
     sys.path.insert(0, str(REPO))
@@ -768,8 +816,8 @@ def cpu_baseline(sc, hw, clip, heights, cam, W, H, rows):
     dt = time.perf_counter() - t
     n = rows * 8 * W
     return {"value": round(n / dt, 1), "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"main_primary over tile rows {r0}..{r0 + rows - 1} ({n} rays) of frame 0 at {W}x{H}, "
-                      f"{dt:.1f} s"}
+            "sample": f"main_primary over tile rows {r0}..{r0 + rows - 1} ({n} rays) of flight frame {frame} at "
+                      f"{W}x{H}, {dt:.1f} s"}
 
 
 if __name__ == "__main__":
