@@ -126,7 +126,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(the path shards by request across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(the path shards by request: one server per GPU, each with a full replica of the world)", file=sys.stderr)
         sys.exit(2)
     import voxhip as vx
     from voxhip import scenes

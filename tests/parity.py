@@ -1,5 +1,5 @@
 """Helpers that run the same frame through the HIP passes and the oracle and
-compare them to the north-star tolerances:
+compare them to the parity tolerances:
   * hit identity (kind, chunk slot, leaf node, voxel, material): bit-exact;
   * depth: |d_gpu - d_ref| <= 1e-5 * |d_ref|;
   * colour (primary colour and post-composite output): <= 1e-3 absolute.
