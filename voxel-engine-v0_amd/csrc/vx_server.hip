@@ -449,6 +449,33 @@ void device_main(vx_server* s) {
   vx_ray* h_rays = nullptr;
   vx_hit* h_hits = nullptr;
   size_t h_cap = 0;
+  // Staging for 4 M rays from the start, doubled when a launch needs more:
+  // reallocating (a synchronising free, and pinning hundreds of MB of host
+  // pages) inside a launch would hold every request in it.
+  auto grow = [&](size_t nr) -> int {
+    if (nr <= h_cap && nr <= d_cap) return VX_OK;
+    const size_t cap = std::max({nr, 2 * std::max(h_cap, d_cap), (size_t)1 << 22});
+    if (h_rays) hipHostFree(h_rays);
+    if (h_hits) hipHostFree(h_hits);
+    if (d_rays) hipFree(d_rays);
+    if (d_hits) hipFree(d_hits);
+    h_rays = nullptr;
+    h_hits = nullptr;
+    d_rays = nullptr;
+    d_hits = nullptr;
+    h_cap = d_cap = 0;
+    if (hipHostMalloc((void**)&h_rays, cap * sizeof(vx_ray), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_hits, cap * sizeof(vx_hit), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&d_rays, cap * sizeof(vx_ray)) != hipSuccess ||
+        hipMalloc((void**)&d_hits, cap * sizeof(vx_hit)) != hipSuccess)
+      return vx_fail(VX_E_NOMEM, "server: ray staging for %zu rays", cap);
+    h_cap = d_cap = cap;
+    return VX_OK;
+  };
+  if (grow(1) != VX_OK) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->last_error = VX_E_NOMEM;
+  }
   for (;;) {
     std::vector<Pending> views, rays;
     std::vector<ChunkBuild> builds;
@@ -479,7 +506,7 @@ void device_main(vx_server* s) {
       // target, half the target's worth of rays (at least one request); the
       // rest waits for the next launch.
       const double cap = s->cfg.latency_target_ms > 0.0f && s->rate > 0.0 ? s->rate * 0.5 * s->cfg.latency_target_ms : 0.0;
-      uint64_t launch_rays = 0;
+      uint64_t launch_rays = 0, query_rays = 0;
       std::deque<Pending> keep;
       std::vector<uint32_t> taken;
       while (!s->inbox.empty()) {
@@ -496,10 +523,11 @@ void device_main(vx_server* s) {
             launch_rays += n;
             views.push_back(std::move(p));
           }
-        } else if (full) {
+        } else if (full || (query_rays > 0 && query_rays + n > h_cap)) {  // ray batches beyond the staging wait
           keep.push_back(std::move(p));
         } else {
           launch_rays += n;
+          query_rays += n;
           rays.push_back(std::move(p));
         }
       }
@@ -550,35 +578,11 @@ void device_main(vx_server* s) {
     size_t nr = 0;
     for (Pending& p : rays) nr += p.rays.size();
     if (!rc && nr) {
-      if (nr > h_cap) {
-        if (h_rays) hipHostFree(h_rays);
-        if (h_hits) hipHostFree(h_hits);
-        h_rays = nullptr;
-        h_hits = nullptr;
-        h_cap = 0;
-        const size_t cap = std::max(nr, (size_t)1 << 20);
-        if (hipHostMalloc((void**)&h_rays, cap * sizeof(vx_ray), hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&h_hits, cap * sizeof(vx_hit), hipHostMallocDefault) != hipSuccess)
-          rc = vx_fail(VX_E_NOMEM, "server: pinned ray staging");
-        else
-          h_cap = cap;
-      }
+      rc = grow(nr);
       size_t at = 0;
       for (Pending& p : rays) {
         if (!rc && !p.rays.empty()) std::memcpy(h_rays + at, p.rays.data(), p.rays.size() * sizeof(vx_ray));
         at += p.rays.size();
-      }
-      if (!rc && nr > d_cap) {
-        if (d_rays) hipFree(d_rays);
-        if (d_hits) hipFree(d_hits);
-        d_rays = nullptr;
-        d_hits = nullptr;
-        d_cap = 0;
-        if (hipMalloc((void**)&d_rays, nr * sizeof(vx_ray)) != hipSuccess ||
-            hipMalloc((void**)&d_hits, nr * sizeof(vx_hit)) != hipSuccess)
-          rc = vx_fail(VX_E_NOMEM, "server: ray buffers");
-        else
-          d_cap = nr;
       }
       if (!rc && hipMemcpyAsync(d_rays, h_rays, nr * sizeof(vx_ray), hipMemcpyHostToDevice, s->qstream) != hipSuccess)
         rc = vx_fail(VX_E_DEVICE, "server: ray upload");
