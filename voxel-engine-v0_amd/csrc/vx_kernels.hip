@@ -65,12 +65,12 @@ __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long
 // Occupancy of the primary-pass kernels: 4 waves per SIMD gives the compiler
 // 128 VGPRs.  The pass needs more than that at its peak (the grass sphere
 // trace inside the rope traversal), so it still spills; timed at 3840x2160
-// the persistent kernel runs 30.9 ms at 4 waves, 32.8 at 3, 37.9 at 6 and
-// 39.4 at 2: fewer spills win until latency hiding runs out.
+// the persistent kernel runs 15.8 ms at 4 waves and 16.6 at 3 (the LDS of a
+// tile allows no more than 4): fewer spills win until latency hiding runs out.
 #define VXD_PRIMARY_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
 // The godray kernels: 6 waves per SIMD (80 VGPRs, a few spills) beat 4
-// (120 VGPRs, none) at 3840x2160, 4.27 against 4.51 ms: their sun-shadow
-// traversals are latency-bound.
+// (120 VGPRs, none) and 8 at 3840x2160, 3.57 against 3.90 and 3.62 ms: their
+// sun-shadow traversals are latency-bound.
 #define VXD_GODRAY_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6, 8)))
 
 // ---------------------------------------------------------------------------
