@@ -76,6 +76,12 @@ def test_device_query_code_matches_oracle(vx):
     got = vxd_cpu.query_rays(world, cam, rays, vx.HIT_DTYPE)
     assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
     assert int(np.sum(ref["kind"] == 1)) > 2000
+    # rays on the traversal's ties: lattice origins, slab-tying and near-parallel directions
+    rays = worlds.boundary_rays()
+    ref = O.query_rays(world, cam, rays, vx.HIT_DTYPE)
+    got = vxd_cpu.query_rays(world, cam, rays, vx.HIT_DTYPE)
+    assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+    assert 0 < int(np.sum(ref["kind"] == 1)) < len(rays)
 
 
 def test_device_float_to_int_conversions():

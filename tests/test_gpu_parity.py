@@ -70,6 +70,46 @@ def test_primary_c0_first_frame(gpu, wind, staged):
     assert min(s["kinds"]) > 0
 
 
+@pytest.mark.parametrize("yaw,pitch", [(0.0, 0.0), (math.pi / 2, -math.pi / 6), (math.pi / 4, -math.pi / 4)])
+def test_primary_camera_on_chunk_corner(gpu, yaw, pitch):
+    """The camera exactly on a chunk corner (x and z on chunk faces, y on a
+    voxel face), looking along an axis, an axis-aligned 30-degree dive and a
+    diagonal: every ray starts on the ties of the inward-biased chunk coordinate
+    (common.wgsl:543-554) and the probe rays' chunk DDA; 256x144 primary pass
+    against the oracle."""
+    O = _oracle()
+    from voxhip import scenes
+
+    sc, hw = host_world("c0")
+    pose0, t0 = sc.poses[0]
+    f = np.float32
+    cx, cz = (round(float(pose0.pos[i]) / 6.4) for i in (0, 2))
+    pos = (float(f(cx) * f(6.4)), float(f(round(float(pose0.pos[1]) / 0.1)) * f(0.1)), float(f(cz) * f(6.4)))
+    res = (256, 144)
+    cam = gpu.camera_build(gpu.make_pose(pos, yaw, pitch), res, res, 0, time_s=t0, grid_origin=sc.grid_origin,
+                           grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
+    clip, heights = scenes.scene_clipmap(sc)
+    ctx = make_context(gpu, hw)
+    ctx.write_clipmap(clip, heights)
+    view = gpu.View(ctx, res, res)
+    view.write_camera(cam)
+    view.render(gpu.PASS_PRIMARY | gpu.PASS_WRITE_HITS | gpu.PASS_END_FRAME)
+    w = O.World.from_host(hw, clip, heights)
+    fr = O.Frame(*res)
+    O.primary(w, cam, fr, gpu.HIT_DTYPE)
+    s = summary(view.read(gpu.OUT_HITS), fr.hits, view.read(gpu.OUT_DEPTH), fr.depth, view.read(gpu.OUT_COLOR),
+                fr.color)
+    s["hist_bit_exact"] = bool(np.array_equal(view.read(gpu.OUT_PRIMARY_HIST).view(np.uint32),
+                                              fr.primary_hist.view(np.uint32)))
+    s["kinds"] = np.bincount(fr.hits["kind"].ravel(), minlength=3).tolist()
+    _report(f"chunk-corner camera {pos} yaw={yaw:.3f} pitch={pitch:.3f}", s)
+    _assert_parity(s)
+    assert s["hist_bit_exact"]
+    assert s["kinds"][1] > 0  # voxel hits
+    view.close()
+    ctx.close()
+
+
 def test_light_gather_after_reupload(gpu):
     """The light gather skips its march where no chunk within reach holds a
     LIGHT leaf; the per-chunk flags it reads are derived on the device after

@@ -48,6 +48,29 @@ def test_query_matches_oracle(gpu):
     assert int(np.sum(ref["kind"] == 1)) > 10_000
 
 
+def test_query_boundary_rays_match_oracle(gpu):
+    """Rays whose origins and directions sit on the traversal's ties (slab
+    ties, exit-face ambiguity and re-descent, inward-biased chunk coordinates,
+    the parallel cut): hit records bit-identical to the oracle's."""
+    from oracle import oracle as O
+
+    sc, hw = host_world("c1")
+    ctx = make_context(gpu, hw)
+    cam = _scene_cam(gpu, sc, hw)
+    rays = worlds.boundary_rays()
+    clip, heights = worlds.empty_clipmap(gpu)
+    ref = O.query_rays(O.World.from_host(hw, clip, heights), cam, rays, gpu.HIT_DTYPE)
+    got = gpu.query_rays(ctx, cam, rays)
+    rows = np.nonzero((got.view(np.uint8).reshape(len(rays), -1) != ref.view(np.uint8).reshape(len(rays), -1))
+                      .any(axis=1))[0]
+    kinds = np.bincount(ref["kind"].ravel(), minlength=2).tolist()
+    print(f"[query] boundary rays: {len(rays)}, kinds {kinds}, mismatched rows {rows[:8].tolist()}", flush=True)
+    assert rows.size == 0, [(rays[r].tolist(), got[r], ref[r]) for r in rows[:3]]
+    # both outcomes occur: rays leaving the ground hit, rays into open sky miss
+    assert 0 < kinds[1] < len(rays)
+    ctx.close()
+
+
 def test_query_interface_edges(gpu):
     sc, hw = host_world("c0")
     ctx = make_context(gpu, hw)

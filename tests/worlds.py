@@ -73,3 +73,42 @@ def empty_clipmap(vx):
     clip = vx.Clipmap()
     clip.levels, clip.res = 0, 256
     return clip, np.zeros((1, 256, 256), np.float32)
+
+
+def boundary_rays():
+    """Origins on chunk corners and voxel-lattice points (fp32 sums of the
+    chunk origin and whole voxels: on or within an ulp of node faces, edges and
+    corners), some inside the ground and some in open air, times directions
+    that tie slabs: axis-aligned, face and body diagonals, and near-parallel
+    ones whose small components sit either side of the 1e-8 parallel cut
+    (common.wgsl:442-444, ray/aabb.wgsl:16-84)."""
+    f = np.float32
+    rng = np.random.default_rng(23)
+    origins = []
+    for _ in range(160):
+        c = rng.integers(0, (16, 4, 16))
+        v = rng.integers(0, 64, 3)
+        lattice = rng.integers(0, 4)  # 0: chunk corner, 1: voxel corner, 2: node-8 corner, 3: one face off-lattice
+        if lattice == 0:
+            v[:] = 0
+        elif lattice == 2:
+            v &= ~7
+        o = c.astype(f) * f(6.4) + v.astype(f) * f(0.1)
+        if lattice == 3:
+            o[int(rng.integers(0, 3))] += f(0.05)
+        origins.append(o.astype(f))
+    dirs = []
+    for s in [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+              (1, 1, 0), (1, -1, 0), (-1, 0, -1), (0, -1, 1), (1, 1, 1), (-1, -1, 1), (1, -1, -1), (-1, -1, -1),
+              (1, 5e-9, 0), (1, -5e-9, 2e-8), (3e-9, -1, -3e-9), (2e-8, -1, 0), (1, -1, 1e-9), (0.5, -1, 0.25)]:
+        d = np.array(s, np.float32)
+        dirs.append((d / np.linalg.norm(d)).astype(f))
+    rays = np.zeros((len(origins) * len(dirs), 8), np.float32)
+    k = 0
+    for o in origins:
+        for d in dirs:
+            rays[k, 0:3] = o
+            rays[k, 3] = 51.2 if k % 3 else 200.0
+            rays[k, 4:7] = d
+            k += 1
+    return rays
