@@ -132,6 +132,10 @@ def test_device_transcendentals_match_oracle():
     bad = np.argwhere(~same)
     assert bad.size == 0, [(float(x[i]), float(y[i]), ["sin", "cos", "exp", "log", "pow"][j], float(got[i, j]),
                             float(want[i, j])) for i, j in bad[:8]]
+    # The godray cloud transmittance returns 1 without calling exp when the
+    # density is zero: exact only because exp(+0) and exp(-0) are exactly 1.
+    zeros = vxd_cpu.math(np.array([0.0, -0.0], np.float32), np.ones(2, np.float32))
+    assert zeros[:, 2].view(np.uint32).tolist() == [0x3F800000, 0x3F800000]
 
 
 def test_device_value_noise_matches_oracle():
