@@ -121,6 +121,51 @@ def test_admission_budget(gpu):
     srv.close()
 
 
+def test_close_stream_while_streams_open(gpu):
+    """vx_server_close_stream waits for the stream's queued frames with the lock
+    released; meanwhile another thread opens 64 new streams (the stream map
+    grows and rehashes).  The close completes, every frame completes with
+    status 0, and the other streams stay usable and close cleanly."""
+    import threading
+    import time
+
+    from voxhip import scenes
+
+    sc, hw = host_world("c0")
+    clip, heights = scenes.scene_clipmap(sc)
+    ctx = _ctx(gpu, hw, clip, heights)
+    srv = gpu.Server(ctx, hw, memory_budget=8 << 30)
+    big, small = (320, 180), (32, 18)
+    cams_big = _cams(gpu, sc, hw, big, 4)
+    cam_small = _cams(gpu, sc, hw, small, 1)[0]
+    for f, cam in enumerate(cams_big):
+        assert srv.submit_view(1, cam, *big, tag=f)
+    errors = []
+
+    def close_one():
+        try:
+            srv.close_stream(1)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = threading.Thread(target=close_one)
+    th.start()
+    time.sleep(0.002)
+    for k in range(64):
+        assert srv.submit_view(100 + k, cam_small, *small, tag=100 + k)
+    th.join(timeout=60)
+    assert not th.is_alive() and not errors, errors
+    res = _wait_all(srv, 4 + 64)
+    assert all(r.status == 0 for r in res)
+    st = srv.stats()
+    assert st["completed"] == st["admitted"] == 68
+    assert srv.submit_view(100, cam_small, *small, tag=200)  # an open stream still renders
+    _wait_all(srv, 1)
+    for k in range(64):
+        srv.close_stream(100 + k)
+    srv.close()
+
+
 def test_latency_target_admission(gpu):
     """With a latency target, admission rejects a request once the server's
     measured throughput says the queue and the request cannot finish within

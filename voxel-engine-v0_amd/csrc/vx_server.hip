@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -144,7 +145,8 @@ struct vx_server {
   vx_server_config cfg{};
   hipStream_t qstream = nullptr;
   // world
-  std::vector<vx_chunk_meta> meta;  // host mirror of the device meta table
+  std::vector<vx_chunk_meta> meta;  // host mirror of the device meta table (device thread only after create)
+  std::vector<std::array<int32_t, 3>> origin;  // per slot: chunk origin in voxels (immutable, read by workers)
   Arena arena;
   std::unordered_map<uint32_t, std::unordered_map<uint32_t, uint8_t>> edits;  // slot -> voxel index -> material
   std::vector<uint8_t> dirty;        // slot queued for rebuild
@@ -213,7 +215,6 @@ void worker_main(vx_server* s) {
   std::vector<uint8_t> mat(64 * 64 * 64);
   std::vector<int32_t> ground(64 * 64);
   ChunkBuild b;
-  int32_t origin[3] = {0, 0, 0};
   b.nodes.resize(VX_MAX_CHUNK_NODES);
   b.ropes.resize(VX_MAX_CHUNK_NODES);
   for (;;) {
@@ -229,10 +230,8 @@ void worker_main(vx_server* s) {
       b.gen = s->gen[slot];
       ++s->building;
       for (auto& kv : s->edits[slot]) ed.emplace_back(kv.first, kv.second);
-      origin[0] = s->meta[slot].origin[0];
-      origin[1] = s->meta[slot].origin[1];
-      origin[2] = s->meta[slot].origin[2];
     }
+    const std::array<int32_t, 3>& origin = s->origin[slot];
     const int32_t cx = origin[0] / 64, cy = origin[1] / 64, cz = origin[2] / 64;
     uint32_t count = 0;
     int rc = vx_world_fill_chunk(&s->cfg.world, cx, cy, cz, mat.data(), ground.data());
@@ -670,6 +669,9 @@ int vx_server_create(vx_ctx* ctx, const vx_host_world_view* world, const vx_serv
   if (!s->cfg.max_views_per_launch) s->cfg.max_views_per_launch = 64;
   if (!s->cfg.edit_threads) s->cfg.edit_threads = 8;
   s->meta.assign(world->meta, world->meta + world->chunk_count);
+  s->origin.resize(world->chunk_count);
+  for (uint32_t i = 0; i < world->chunk_count; ++i)
+    s->origin[i] = {world->meta[i].origin[0], world->meta[i].origin[1], world->meta[i].origin[2]};
   s->dirty.assign(world->chunk_count, 0);
   s->gen.assign(world->chunk_count, 0);
   s->applied.assign(world->chunk_count, 0);
@@ -843,12 +845,17 @@ int vx_server_close_stream(vx_server* s, uint32_t stream_id) {
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->streams.find(stream_id);
     if (it == s->streams.end()) return vx_fail(VX_E_INVALID, "vx_server_close_stream: no stream %u", stream_id);
-    // wait until no frame of the stream is queued or in flight
+    // Wait until no frame of the stream is queued or in flight.  The wait
+    // releases the lock: submits may rehash the map and another close may
+    // erase the stream meanwhile, so look it up again afterwards.
     s->cv.wait(lk, [&] {
       for (auto& p : s->inbox)
         if (p.req.kind == VX_REQ_VIEW && p.req.stream_id == stream_id) return false;
-      return !s->streams[stream_id].busy;
+      const auto cur = s->streams.find(stream_id);
+      return cur == s->streams.end() || !cur->second.busy;
     });
+    it = s->streams.find(stream_id);
+    if (it == s->streams.end()) return vx_fail(VX_E_INVALID, "vx_server_close_stream: stream %u closed meanwhile", stream_id);
     v = it->second.view;
     s->mem_in_use -= it->second.bytes;
     s->streams.erase(it);
