@@ -80,6 +80,11 @@ def test_write_clipmap_rects_rejects_bad_rectangles(gpu):
         ctx.write_clipmap_rects(clip, sc.params, [bad])
     with pytest.raises(gpu.VoxhipError, match="rectangles"):
         ctx.write_clipmap_rects(clip, sc.params, rects * 40)
+    # host patches whose x + w wraps in 32 bits are rejected before any copy
+    bad = gpu.ClipRect.from_buffer_copy(rects[0])
+    bad.x, bad.w, bad.h = 0xFFFFFFF0, 32, 1
+    with pytest.raises(gpu.VoxhipError, match="out of range"):
+        ctx.write_clipmap_patches(clip, [bad], [np.zeros(32, np.float32)])
     # the context is still usable
     ctx.write_clipmap_rects(clip, sc.params, rects)
     h = ctx.read_clipmap()

@@ -387,7 +387,8 @@ int vx_write_clipmap(vx_ctx* ctx, const vx_clipmap* params, const vx_clip_patch*
                    ctx->clip_levels, ctx->clip_res);
   for (uint32_t i = 0; i < n; ++i) {
     const vx_clip_patch& p = patches[i];
-    if (p.level >= ctx->clip_levels || p.x + p.w > ctx->clip_res || p.y + p.h > ctx->clip_res || (!p.data && p.w && p.h))
+    if (p.level >= ctx->clip_levels || p.x > ctx->clip_res || p.w > ctx->clip_res - p.x || p.y > ctx->clip_res ||
+        p.h > ctx->clip_res - p.y || (!p.data && p.w && p.h))
       return vx_fail(VX_E_INVALID, "vx_write_clipmap: patch %u out of range", i);
   }
   DeviceGuard g(ctx->device);
@@ -479,6 +480,7 @@ void vx_view_destroy(vx_view* v) {
 
 int vx_view_resize(vx_view* v, uint32_t pw, uint32_t ph, uint32_t rw, uint32_t rh) {
   if (!v || !pw || !ph || !rw || !rh) return vx_fail(VX_E_INVALID, "vx_view_resize: bad argument");
+  if ((uint64_t)pw * ph > (1ull << 28) || (uint64_t)rw * rh > (1ull << 28)) return vx_fail(VX_E_INVALID, "vx_view_resize: too large");
   DeviceGuard g(v->ctx->device);
   VX_HIP(hipStreamSynchronize(v->ctx->stream));
   free_view_buffers(v);
