@@ -286,7 +286,10 @@ int vx_view_timings(vx_view* view, float ms[5]);
  * without waiting.  All n rays run as ONE persistent work-stealing launch in
  * which lanes whose ray finished take new rays (lane refill).  Hit: kind 1,
  * slot, leaf node (chunk-local), voxel (world voxel coords of the leaf's
- * minimum corner), mat, t.  Miss: kind 0, slot/node 0xFFFFFFFF. */
+ * minimum corner), mat, t.  Miss: kind 0, slot/node 0xFFFFFFFF.
+ * A context has one query camera buffer and one ray cursor: queries issued on
+ * one stream are ordered by it, but a query on another stream must not start
+ * before the previous one has finished (they would race on both). */
 typedef struct vx_ray {
     float ox, oy, oz, tmax;   /* origin (metres), max distance (metres) */
     float dx, dy, dz, pad;    /* direction (unit) */
