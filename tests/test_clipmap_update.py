@@ -208,3 +208,9 @@ def test_rejects_small_rect_buffer(vx):
     # nothing changed: the next update is still the first touch
     _, r = st.update(0.0, 0.0, 0.0)
     assert len(r) == 4 and all(x.w == RES and x.h == RES for x in r)
+
+
+@pytest.mark.parametrize("cell", [0.0, -1.0, float("nan"), float("inf")])
+def test_build_rejects_bad_cell(vx, world, cell):
+    with pytest.raises(vx.VoxhipError, match="bad argument"):
+        vx.clipmap_build(world, 0.0, 0.0, levels=2, res=RES, base_cell_m=cell)

@@ -808,7 +808,8 @@ int vx_camera_build(const vx_pose* pose, uint32_t render_w, uint32_t render_h, u
 // ---------------------------------------------------------------------------
 int vx_clipmap_build(const vx_world_params* p, float cam_x, float cam_z, uint32_t levels, uint32_t res,
                      float base_cell_m, vx_clipmap* out, float* heights) {
-  if (!p || !out || !heights || levels > VX_CLIP_LEVELS_MAX || res == 0 || res > 4096)
+  if (!p || !out || !heights || levels > VX_CLIP_LEVELS_MAX || res == 0 || res > 4096 || !(base_cell_m > 0.0f) ||
+      !std::isfinite(base_cell_m))
     return vx_fail(VX_E_INVALID, "vx_clipmap_build: bad argument");
   std::memset(out, 0, sizeof(*out));
   out->levels = levels;
@@ -891,7 +892,7 @@ extern "C" {
 int vx_clipmap_state_create(uint32_t levels, uint32_t res, float base_cell_m, float min_update_interval_s,
                             uint32_t flags, vx_clipmap_state** out) {
   if (!out || levels == 0 || levels > VX_CLIP_LEVELS_MAX || res < 2 * VX_CLIP_EDGE_GUARD + 1 || res > 4096 ||
-      !(base_cell_m > 0.0f) || (flags & ~VX_CLIP_WRAP_FIX))
+      !(base_cell_m > 0.0f) || !std::isfinite(base_cell_m) || (flags & ~VX_CLIP_WRAP_FIX))
     return vx_fail(VX_E_INVALID, "vx_clipmap_state_create: bad argument");
   vx_clipmap_state* s = new (std::nothrow) vx_clipmap_state();
   if (!s) return vx_fail(VX_E_NOMEM, "vx_clipmap_state_create: host allocation");
