@@ -145,7 +145,8 @@ def test_light_gather_after_reupload(gpu):
         assert (s["lit_pixels"] > 0) == (name == "lit")
 
 
-def test_full_pipeline_multiframe(gpu):
+@pytest.mark.parametrize("fog", [0.002, 0.0])
+def test_full_pipeline_multiframe(gpu, fog):
     """Four full frames (primary, local TAA, godray, composite, composite TAA)
     with a moving camera, so history reprojection, anchors and the 1/8 shadow
     refresh feed back frame to frame on both sides."""
@@ -165,8 +166,9 @@ def test_full_pipeline_multiframe(gpu):
     for f in range(4):
         pose = gpu.make_pose((pose0.pos[0] + 0.05 * f, pose0.pos[1], pose0.pos[2] - 0.03 * f), pose0.yaw + 0.01 * f,
                              pose0.pitch, 70.0)
-        cam = gpu.camera_build(pose, res, res, f, time_s=f / 60.0, wind_strength=2.0, grid_origin=sc.grid_origin,
-                               grid_dims=sc.grid_dims, chunk_count=hw.chunk_count, prev_view_proj=prev, jitter=True)
+        cam = gpu.camera_build(pose, res, res, f, time_s=f / 60.0, wind_strength=2.0, fog_density=fog,
+                               grid_origin=sc.grid_origin, grid_dims=sc.grid_dims, chunk_count=hw.chunk_count,
+                               prev_view_proj=prev, jitter=True)
         prev = np.array(cam.view_proj, np.float32)
         view.write_camera(cam)
         view.render(gpu.PASS_FRAME | gpu.PASS_WRITE_HITS)
@@ -177,8 +179,10 @@ def test_full_pipeline_multiframe(gpu):
         s["output_max_abs"] = float(np.max(np.abs(out - fr.output)))
         s["godray_max_abs"] = float(np.max(np.abs(view.read(gpu.OUT_GODRAY) - fr.godray)))
         s["local_hist_max_abs"] = float(np.max(np.abs(view.read(gpu.OUT_LOCAL_HIST) - fr.local_hist)))
-        _report(f"pipeline frame {f}", s)
+        _report(f"pipeline frame {f} fog={fog}", s)
         _assert_parity(s)
+        if fog == 0.0:  # no fog: the godray march is off and its output is exactly zero
+            assert not view.read(gpu.OUT_GODRAY)[..., :3].any()
         assert s["output_max_abs"] <= COLOR_ABS_TOL
         assert s["godray_max_abs"] <= COLOR_ABS_TOL
         assert s["local_hist_max_abs"] <= COLOR_ABS_TOL
