@@ -258,14 +258,15 @@ struct EmitArgs {
 
 // Rope target one level down (builder_ropes.rs:31-75): an invalid rope stays
 // invalid, a leaf neighbour is the target itself, an interior neighbour gives
-// its child on the near side (or invalid when that child is absent).
-__device__ inline uint32_t descend_one(const vx_node* nodes, uint32_t nei, uint32_t ci) {
+// its child on the near side (or invalid when that child is absent).  The
+// neighbour's child_base and child_mask are passed in: a parent reads its six
+// rope targets' nodes once for all of its children.
+__device__ inline uint32_t descend_one(uint32_t nei, uint32_t nbase, uint32_t nmask, uint32_t ci) {
   if (nei == kInvalid) return kInvalid;
-  const vx_node n = nodes[nei];
-  if (n.child_base == kInvalid) return nei;
+  if (nbase == kInvalid) return nei;
   const uint32_t bit = 1u << ci;
-  if (!(n.child_mask & bit)) return kInvalid;
-  return n.child_base + (uint32_t)__popc(n.child_mask & (bit - 1u));
+  if (!(nmask & bit)) return kInvalid;
+  return nbase + (uint32_t)__popc(nmask & (bit - 1u));
 }
 
 __global__ __launch_bounds__(kThreads) void k_build_emit(EmitArgs a) {
@@ -313,6 +314,20 @@ __global__ __launch_bounds__(kThreads) void k_build_emit(EmitArgs a) {
       const bool visited = (pw & kVisited) != 0;
       vx_node_ropes pr = {kInvalid, kInvalid, kInvalid, kInvalid, kInvalid, kInvalid, 0u, 0u};
       if (visited) pr = ropes[pw & ~kVisited];
+      // The rope targets are nodes of earlier levels (complete behind the level
+      // barrier): their words are loaded together, not once per child between
+      // the children's stores.
+      const uint32_t nei[6] = {pr.px, pr.nx, pr.py, pr.ny, pr.pz, pr.nz};
+      uint32_t nbase[6], nmask[6];
+      for (uint32_t d = 0; d < 6; ++d) {
+        nbase[d] = kInvalid;
+        nmask[d] = 0u;
+        if (nei[d] != kInvalid) {
+          const vx_node nn = nodes[nei[d]];
+          nbase[d] = nn.child_base;
+          nmask[d] = nn.child_mask;
+        }
+      }
       uint32_t kid[8];
       uint32_t rank = 0;
       for (uint32_t ci = 0; ci < 8; ++ci) kid[ci] = (pmask >> ci) & 1u ? first + pscan[p] + rank++ : kInvalid;
@@ -336,12 +351,12 @@ __global__ __launch_bounds__(kThreads) void k_build_emit(EmitArgs a) {
         vx_node_ropes r = {kInvalid, kInvalid, kInvalid, kInvalid, kInvalid, kInvalid, 0u, 0u};
         if (visited) {
           const uint32_t sx = kid[ci ^ 1u], sy = kid[ci ^ 2u], sz = kid[ci ^ 4u];
-          r.px = (hx == 0 && sx != kInvalid) ? sx : descend_one(nodes, pr.px, 0u | (hy << 1) | (hz << 2));
-          r.nx = (hx == 1 && sx != kInvalid) ? sx : descend_one(nodes, pr.nx, 1u | (hy << 1) | (hz << 2));
-          r.py = (hy == 0 && sy != kInvalid) ? sy : descend_one(nodes, pr.py, hx | (0u << 1) | (hz << 2));
-          r.ny = (hy == 1 && sy != kInvalid) ? sy : descend_one(nodes, pr.ny, hx | (1u << 1) | (hz << 2));
-          r.pz = (hz == 0 && sz != kInvalid) ? sz : descend_one(nodes, pr.pz, hx | (hy << 1) | (0u << 2));
-          r.nz = (hz == 1 && sz != kInvalid) ? sz : descend_one(nodes, pr.nz, hx | (hy << 1) | (1u << 2));
+          r.px = (hx == 0 && sx != kInvalid) ? sx : descend_one(nei[0], nbase[0], nmask[0], 0u | (hy << 1) | (hz << 2));
+          r.nx = (hx == 1 && sx != kInvalid) ? sx : descend_one(nei[1], nbase[1], nmask[1], 1u | (hy << 1) | (hz << 2));
+          r.py = (hy == 0 && sy != kInvalid) ? sy : descend_one(nei[2], nbase[2], nmask[2], hx | (0u << 1) | (hz << 2));
+          r.ny = (hy == 1 && sy != kInvalid) ? sy : descend_one(nei[3], nbase[3], nmask[3], hx | (1u << 1) | (hz << 2));
+          r.pz = (hz == 0 && sz != kInvalid) ? sz : descend_one(nei[4], nbase[4], nmask[4], hx | (hy << 1) | (0u << 2));
+          r.nz = (hz == 1 && sz != kInvalid) ? sz : descend_one(nei[5], nbase[5], nmask[5], hx | (hy << 1) | (1u << 2));
         }
         ropes[kid[ci]] = r;
         if (L < 6) idx[level_first(L) + c] = kid[ci] | (visited ? kVisited : 0u);
