@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--no-chunk-builds", action="store_true", help="skip the chunk builder measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed views + ray batches workload")
     ap.add_argument("--only", choices=["mixed", "chunk_builds", "ray_queries", "serving", "serving_device_builds",
-                                       "config0"],
+                                       "serving_1080p", "config0"],
                     help="run one workload alone and print its result (not the benchmark line)")
     ap.add_argument("--static-clipmap", action="store_true",
                     help="keep frame 0's clipmap instead of updating it every frame")
@@ -138,7 +138,8 @@ def main():
                 "ray_queries": lambda: ray_queries(vx, scenes, a.query_batches),
                 "serving": lambda: serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms),
                 "serving_device_builds": lambda: serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms,
-                                                       device_build=True)}
+                                                       device_build=True),
+                "serving_1080p": lambda: serve_1080p(vx, scenes, 4.0)}
         print(json.dumps({a.only: legs[a.only]()}), flush=True)
         return
 
@@ -265,6 +266,7 @@ def main():
         serving_untargeted = serve(vx, scenes, a.serve_seconds, target_ms=0.0)
     if a.serve_seconds > 0:
         serving_device = serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms, device_build=True)
+    serving1080 = None if a.serve_seconds <= 0 else serve_1080p(vx, scenes, 4.0)
     builds = None if a.no_chunk_builds else chunk_builds(vx, scenes)
 
     cpu = None
@@ -306,6 +308,7 @@ def main():
         "serving": serving,
         "serving_without_latency_target": serving_untargeted,
         "serving_device_builds": serving_device,
+        "serving_1080p": serving1080,
         "chunk_builds": builds,
         "setup_s": round(setup_s, 2),
     }
@@ -692,6 +695,77 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
                                 "in_launch": round(float(np.mean(slow[:, 2] - slow[:, 1])), 2),
                                 "launch_ms_p50": round(float(np.percentile(d[:, 2] - d[:, 1], 50)), 2),
                                 "launch_ms_max": round(float(np.max(d[:, 2] - d[:, 1])), 2)}}
+
+
+def serve_1080p(vx, scenes, seconds, streams=(2, 3, 4), fps=30.0, target_ms=16.0):
+    """Interactive serving at 1080p: `n` client view streams of 1920x1080 on the
+    16x4x16-chunk world with a 5-level clipmap, each submitting a frame every
+    1/fps s (phases spread evenly over the period) along its own stretch of the
+    seeded flight, at most one frame pending per stream (a frame whose
+    predecessor is still pending is rejected: the client drops it), under a
+    target_ms latency target; for each n in `streams`, p50/p99 of submit -> done
+    over the served frames."""
+    sc = scenes.c1()
+    hw = vx.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
+    clip, heights = scenes.scene_clipmap(sc, 0)
+    res = (1920, 1080)
+    out = []
+    for n in streams:
+        ctx = vx.Context(node_capacity=len(hw.nodes), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
+                         clip_levels=5, clip_res=256)
+        ctx.upload_world(hw)
+        ctx.write_clipmap(clip, heights)
+        srv = vx.Server(ctx, hw, memory_budget=64 << 30, max_queued_frames=1, latency_target_ms=target_ms)
+        # warm-up: one frame per stream (creates the resident views), not counted
+        for k in range(n):
+            pose, t_s = sc.poses[(37 * k) % len(sc.poses)]
+            srv.submit_view(k, vx.camera_build(pose, res, res, 0, time_s=t_s, grid_origin=sc.grid_origin,
+                                               grid_dims=sc.grid_dims, chunk_count=hw.chunk_count, jitter=True),
+                            *res, tag=k)
+        srv.drain()
+        while srv.poll() is not None:
+            pass
+        base = srv.stats()
+        ticks = sorted((j / fps + k / (fps * n), k, j) for k in range(n) for j in range(int(seconds * fps)))
+        frame_of = [1] * n
+        offered = dropped = 0
+        done = []
+        t0 = time.perf_counter()
+        for ta, k, _ in ticks:
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            f = frame_of[k]
+            pose, t_s = sc.poses[(37 * k + f) % len(sc.poses)]
+            cam = vx.camera_build(pose, res, res, f, time_s=t_s, grid_origin=sc.grid_origin, grid_dims=sc.grid_dims,
+                                  chunk_count=hw.chunk_count, jitter=True)
+            offered += 1
+            if srv.submit_view(k, cam, *res, tag=n + offered):
+                frame_of[k] += 1
+            else:
+                dropped += 1
+            while (r := srv.poll()) is not None:
+                done.append(r.done_ms - r.submit_ms)
+        srv.drain()
+        wall = time.perf_counter() - t0
+        while (r := srv.poll()) is not None:
+            done.append(r.done_ms - r.submit_ms)
+        st = srv.stats()
+        srv.close()
+        ctx.close()
+        lat = np.array(done) if done else np.zeros(1)
+        served = int(st["completed"] - base["completed"])
+        out.append({"streams": n, "offered_fps": round(n * fps, 1), "served_fps": round(served / wall, 1),
+                    "dropped_frames": dropped, "served_frames": served,
+                    "rays_per_s": round(served * res[0] * res[1] / wall, 1),
+                    "launches": int(st["launches"] - base["launches"]),
+                    "view_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                                        "p99": round(float(np.percentile(lat, 99)), 2),
+                                        "max": round(float(np.max(lat)), 2)}})
+    return {"workload": f"1920x1080 view streams at {fps:g} fps each (evenly phased), 16x4x16 chunks + 5-level "
+                        f"clipmap, at most 1 pending frame per stream (a frame whose predecessor is pending is "
+                        f"dropped), {target_ms:g} ms latency target, {seconds:g} s per load; latency = submit -> done",
+            "loads": out}
 
 
 def chunk_builds(vx, scenes, repeats=3, cpu_sample=24):

@@ -54,3 +54,7 @@ int vx_internal_builder_count(vx_chunk_builder* b, uint32_t n, const uint8_t* ma
                               uint8_t* solid);
 // dst[off[i]] = val[i] for device arrays off / val (distinct offsets), on the context stream.
 int vx_internal_scatter_u8(vx_ctx* ctx, uint8_t* dst, const uint64_t* d_off, const uint8_t* d_val, uint32_t n);
+// Points an existing batch at another set of views of its context (the same
+// checks as vx_batch_create), reusing its device arrays when they are large
+// enough.  The previous render of the batch must have completed.
+int vx_internal_batch_rebind(vx_batch* b, vx_view* const* views, uint32_t n);

@@ -532,6 +532,8 @@ struct vx_batch {
   vx_batch_config cfg{6u, 1u, 1u, 0u};
   std::map<uint64_t, hipGraphExec_t> graphs;  // by (mask, config)
   hipEvent_t ev[2] = {nullptr, nullptr};
+  uint32_t cap = 0;                      // views the device table and ping arrays hold
+  std::vector<vxd::BatchView> tab_host;  // the table as last uploaded
 };
 
 namespace {
@@ -546,6 +548,14 @@ uint32_t persistent_groups(const vx_batch_config& cfg) {
 void drop_graphs(vx_batch* b) {
   for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
   b->graphs.clear();
+}
+int check_batch_views(const vx_ctx* ctx, vx_view* const* views, uint32_t n, const char* fn) {
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!views[i] || views[i]->ctx != ctx) return vx_fail(VX_E_INVALID, "%s: view %u is null or of another context", fn, i);
+    for (uint32_t j = 0; j < i; ++j)
+      if (views[j] == views[i]) return vx_fail(VX_E_INVALID, "%s: view %u appears twice", fn, i);
+  }
+  return VX_OK;
 }
 }  // namespace
 
@@ -576,6 +586,67 @@ vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits) {
   fb.output = v->output;
   fb.hits = hits ? v->hits : nullptr;
   return fb;
+}
+
+// Binds a batch to `views`: the view table (largest render first: their tiles
+// are taken first, small views fill the tail) and the ping mirror, uploaded
+// before this returns.  The device arrays grow only when n exceeds their
+// capacity; graphs captured for the old table are dropped.
+int batch_bind(vx_batch* b, vx_view* const* views, uint32_t n) {
+  drop_graphs(b);
+  b->views.assign(views, views + n);
+  std::stable_sort(b->views.begin(), b->views.end(),
+                   [](const vx_view* a, const vx_view* c) { return (uint64_t)a->rw * a->rh > (uint64_t)c->rw * c->rh; });
+  b->tab_host.resize(n);
+  b->gens.clear();
+  b->ping_host.clear();
+  uint32_t tiles = 0, rblks = 0, pblks = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_view* v = b->views[i];
+    vxd::BatchView& e = b->tab_host[i];
+    std::memset(&e, 0, sizeof(e));
+    e.fb[0] = view_buffers(v, 0, false);
+    e.fb[1] = view_buffers(v, 1, false);
+    e.cam = v->cam_dev;
+    e.tiles_x = (v->rw + 7u) / 8u;
+    e.tiles = e.tiles_x * ((v->rh + 7u) / 8u);
+    e.rblk_x = (v->rw + 15u) / 16u;
+    e.rblks = e.rblk_x * ((v->rh + 15u) / 16u);
+    e.pblk_x = (v->pw + 15u) / 16u;
+    e.pblks = e.pblk_x * ((v->ph + 15u) / 16u);
+    e.tile_base = tiles;
+    e.rblk_base = rblks;
+    e.pblk_base = pblks;
+    tiles += e.tiles;
+    rblks += e.rblks;
+    pblks += e.pblks;
+    b->gens.push_back(v->gen);
+    b->ping_host.push_back((uint32_t)v->ping);
+  }
+  if (n > b->cap) {
+    dfree(b->table);
+    dfree(b->ping);
+    b->table = nullptr;
+    b->ping = nullptr;
+    b->cap = 0;
+    int rc = dalloc(&b->table, n);
+    if (!rc) rc = dalloc(&b->ping, n);
+    if (rc) return rc;
+    b->cap = n;
+  }
+  hipStream_t st = b->ctx->stream;
+  VX_HIP(hipMemcpyAsync(b->table, b->tab_host.data(), n * sizeof(vxd::BatchView), hipMemcpyHostToDevice, st));
+  VX_HIP(hipMemcpyAsync(b->ping, b->ping_host.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  VX_HIP(hipStreamSynchronize(st));  // a render may run on another stream
+  b->B.views = b->table;
+  b->B.ping = b->ping;
+  b->B.cursor = b->cursor;
+  b->B.n = n;
+  b->B.tiles = tiles;
+  b->B.rblks = rblks;
+  b->B.pblks = pblks;
+  b->B.grab = b->cfg.grab;
+  return VX_OK;
 }
 
 vxd::BatchView one_view_entry(const vx_view* v, bool hits) {
@@ -761,64 +832,20 @@ int vx_view_timings(vx_view* v, float ms[5]) {
 int vx_batch_create(vx_ctx* ctx, vx_view* const* views, uint32_t n, vx_batch** out) {
   if (!ctx || !views || !n || !out) return vx_fail(VX_E_INVALID, "vx_batch_create: bad argument");
   *out = nullptr;
-  for (uint32_t i = 0; i < n; ++i) {
-    if (!views[i] || views[i]->ctx != ctx) return vx_fail(VX_E_INVALID, "vx_batch_create: view %u is null or of another context", i);
-    for (uint32_t j = 0; j < i; ++j)
-      if (views[j] == views[i]) return vx_fail(VX_E_INVALID, "vx_batch_create: view %u appears twice", i);
-  }
+  int rc = check_batch_views(ctx, views, n, "vx_batch_create");
+  if (rc) return rc;
   DeviceGuard g(ctx->device);
   vx_batch* b = new (std::nothrow) vx_batch();
   if (!b) return vx_fail(VX_E_NOMEM, "vx_batch_create: host allocation");
   b->ctx = ctx;
-  b->views.assign(views, views + n);
-  // Largest views first: their tiles are taken first, small views fill the tail.
-  std::stable_sort(b->views.begin(), b->views.end(),
-                   [](const vx_view* a, const vx_view* c) { return (uint64_t)a->rw * a->rh > (uint64_t)c->rw * c->rh; });
-  std::vector<vxd::BatchView> tab(n);
-  uint32_t tiles = 0, rblks = 0, pblks = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    const vx_view* v = b->views[i];
-    vxd::BatchView& e = tab[i];
-    std::memset(&e, 0, sizeof(e));
-    e.fb[0] = view_buffers(v, 0, false);
-    e.fb[1] = view_buffers(v, 1, false);
-    e.cam = v->cam_dev;
-    e.tiles_x = (v->rw + 7u) / 8u;
-    e.tiles = e.tiles_x * ((v->rh + 7u) / 8u);
-    e.rblk_x = (v->rw + 15u) / 16u;
-    e.rblks = e.rblk_x * ((v->rh + 15u) / 16u);
-    e.pblk_x = (v->pw + 15u) / 16u;
-    e.pblks = e.pblk_x * ((v->ph + 15u) / 16u);
-    e.tile_base = tiles;
-    e.rblk_base = rblks;
-    e.pblk_base = pblks;
-    tiles += e.tiles;
-    rblks += e.rblks;
-    pblks += e.pblks;
-    b->gens.push_back(v->gen);
-    b->ping_host.push_back((uint32_t)v->ping);
-  }
-  int rc = dalloc(&b->table, n);
-  if (!rc) rc = dalloc(&b->ping, n);
-  if (!rc) rc = dalloc(&b->cursor, 2u * vxd::kBatchBands);
+  rc = dalloc(&b->cursor, 2u * vxd::kBatchBands);
   for (int i = 0; i < 2 && !rc; ++i)
     if (hipEventCreate(&b->ev[i]) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "vx_batch_create: hipEventCreate");
-  if (!rc && hipMemcpy(b->table, tab.data(), n * sizeof(vxd::BatchView), hipMemcpyHostToDevice) != hipSuccess)
-    rc = vx_fail(VX_E_DEVICE, "vx_batch_create: table upload");
-  if (!rc && hipMemcpy(b->ping, b->ping_host.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
-    rc = vx_fail(VX_E_DEVICE, "vx_batch_create: ping upload");
+  if (!rc) rc = batch_bind(b, views, n);
   if (rc) {
     vx_batch_destroy(b);
     return rc;
   }
-  b->B.views = b->table;
-  b->B.ping = b->ping;
-  b->B.cursor = b->cursor;
-  b->B.n = n;
-  b->B.tiles = tiles;
-  b->B.rblks = rblks;
-  b->B.pblks = pblks;
-  b->B.grab = b->cfg.grab;
   *out = b;
   return VX_OK;
 }
@@ -1038,3 +1065,11 @@ int vx_query_tune(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_de
 }
 
 }  // extern "C"
+
+int vx_internal_batch_rebind(vx_batch* b, vx_view* const* views, uint32_t n) {
+  if (!b || !views || !n) return vx_fail(VX_E_INVALID, "vx_batch rebind: bad argument");
+  int rc = check_batch_views(b->ctx, views, n, "vx_batch rebind");
+  if (rc) return rc;
+  DeviceGuard g(b->ctx->device);
+  return batch_bind(b, views, n);
+}

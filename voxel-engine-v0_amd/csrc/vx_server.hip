@@ -475,6 +475,9 @@ void device_main(vx_server* s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->last_error = VX_E_NOMEM;
   }
+  // One batch for the thread's lifetime, re-bound to each launch's views: no
+  // allocation, event creation or synchronising free per launch.
+  vx_batch* batch = nullptr;
   for (;;) {
     std::vector<Pending> views, rays;
     std::vector<ChunkBuild> builds;
@@ -566,12 +569,17 @@ void device_main(vx_server* s) {
       if (!rc) rc = vx_write_camera(stp->view, &p.req.camera);
       vs.push_back(stp->view);
     }
-    vx_batch* batch = nullptr;
+    bool rendered = false;
     if (!rc && !vs.empty()) {
-      rc = vx_batch_create(s->ctx, vs.data(), (uint32_t)vs.size(), &batch);
-      vx_batch_config bc{6u, 1u, 0u, 0u};  // the view set changes per launch: direct launches
-      if (!rc) rc = vx_batch_set_config(batch, &bc);
+      if (!batch) {
+        rc = vx_batch_create(s->ctx, vs.data(), (uint32_t)vs.size(), &batch);
+        vx_batch_config bc{6u, 1u, 0u, 0u};  // the view set changes per launch: direct launches
+        if (!rc) rc = vx_batch_set_config(batch, &bc);
+      } else {
+        rc = vx_internal_batch_rebind(batch, vs.data(), (uint32_t)vs.size());
+      }
       if (!rc) rc = vx_batch_render(batch, VX_PASS_FRAME, nullptr);
+      rendered = true;
     }
     // Ray batches: one fused query launch on the second stream.
     size_t nr = 0;
@@ -597,7 +605,7 @@ void device_main(vx_server* s) {
         off += p.rays.size();
       }
     }
-    if (!rc && batch) rc = vx_sync(s->ctx);
+    if (!rc && rendered) rc = vx_sync(s->ctx);
     if (!rc && nr && hipStreamSynchronize(s->qstream) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "server: query stream");
     // Outputs and results.
     for (size_t i = 0; i < views.size() && !rc; ++i)
@@ -609,7 +617,6 @@ void device_main(vx_server* s) {
       if (!rc && p.req.out_hits) std::memcpy(p.req.out_hits, h_hits + at, p.rays.size() * sizeof(vx_hit));
       at += p.rays.size();
     }
-    if (batch) vx_batch_destroy(batch);
     const double t1 = now_ms();
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -648,6 +655,7 @@ void device_main(vx_server* s) {
     }
     s->cv.notify_all();
   }
+  if (batch) vx_batch_destroy(batch);
   if (d_rays) hipFree(d_rays);
   if (d_hits) hipFree(d_hits);
   if (h_rays) hipHostFree(h_rays);
