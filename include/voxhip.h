@@ -40,7 +40,7 @@ extern "C" {
 #define VX_E_DEVICE     -3  /* HIP runtime error                             */
 #define VX_E_NOMEM      -4  /* device or host allocation failed              */
 #define VX_E_NODEVICE   -5  /* no usable GPU                                 */
-#define VX_E_BUSY       -6  /* serving: request rejected by admission        */
+#define VX_E_BUSY       -6  /* rejected by admission; vx_destroy: live children */
 #define VX_E_AGAIN      -7  /* serving: nothing finished yet                 */
 
 #define VX_INVALID_U32  0xFFFFFFFFu
@@ -164,9 +164,15 @@ typedef struct vx_caps {
  * one kernel per 8x8 tile (same results; a scheduling choice). */
 #define VX_CTX_PRIMARY_STAGED   2u
 
-/* Renderer::new (render/state/mod.rs:147-268): allocates the resident world. */
-int  vx_create(const vx_caps* caps, vx_ctx** out);
-void vx_destroy(vx_ctx* ctx);
+/* Renderer::new (render/state/mod.rs:147-268): allocates the resident world.
+ * vx_destroy frees it (the Renderer's drop).  Teardown order: every object
+ * created on the context (views, batches, servers, chunk builders) must be
+ * destroyed first, because each of them launches on the context's stream and
+ * reads its arenas (a server's device thread keeps doing so until
+ * vx_server_destroy joins it).  While any of them is alive vx_destroy frees
+ * nothing and returns VX_E_BUSY; the context stays valid. */
+int vx_create(const vx_caps* caps, vx_ctx** out);
+int vx_destroy(vx_ctx* ctx);
 
 /* Arenas addressed by vx_upload_regions (render/state/mod.rs:514-582). */
 #define VX_ARENA_NODES   0
@@ -569,7 +575,8 @@ typedef struct vx_request {
     uint32_t kind;                   /* VX_REQ_VIEW or VX_REQ_RAYS */
     uint32_t stream_id;              /* VIEW: the client's view stream */
     uint32_t width, height;          /* VIEW: render = present resolution */
-    vx_camera camera;                /* VIEW: this frame's camera; RAYS: the scene camera */
+    vx_camera camera;                /* VIEW: this frame's camera; RAYS: the scene camera (ray
+                                        batches whose scene fields differ run as separate launches) */
     const vx_ray* rays;              /* RAYS: host array, copied at submit */
     uint32_t n_rays, pad;
     float*   out_rgba;               /* VIEW: optional host buffer, width*height RGBA32F final frame */
@@ -588,6 +595,8 @@ typedef struct vx_server_stats {
     double   busy_ms, wall_ms;
     uint64_t memory_in_use, pad2;
 } vx_server_stats;
+/* The server counts as a live child of ctx until vx_server_destroy, which
+ * stops and joins its threads (vx_destroy refuses before that). */
 int  vx_server_create(vx_ctx* ctx, const vx_host_world_view* world, const vx_server_config* cfg, vx_server** out);
 void vx_server_destroy(vx_server* server);
 /* Queues a request (VX_E_BUSY when admission fails); never blocks on the GPU. */
@@ -596,9 +605,14 @@ int  vx_server_submit(vx_server* server, const vx_request* req);
 int  vx_server_poll(vx_server* server, vx_result* out);
 /* Single-voxel edits (world voxel coords, material). */
 int  vx_server_edit(vx_server* server, const int32_t* xyz, const uint8_t* mats, uint32_t n);
-/* Waits until every queued edit is on the device. */
+/* Waits until every queued edit is on the device.  Returns early with the
+ * server's error when a rebuild, upload or launch failed (errors are sticky:
+ * later waits return the same code), and with VX_E_CAPACITY when the only
+ * edits left are rebuilt chunks the node arena cannot place and nothing else
+ * is queued that could free room. */
 int  vx_server_wait_edits(vx_server* server);
-/* Waits until every admitted request has finished. */
+/* Waits until every admitted request has finished, or returns the server's
+ * sticky error as soon as one is set. */
 int  vx_server_drain(vx_server* server);
 /* Ends a view stream (after its queued frames) and releases its view. */
 int  vx_server_close_stream(vx_server* server, uint32_t stream_id);

@@ -448,6 +448,7 @@ int vx_chunk_builder_create(vx_ctx* ctx, uint32_t max_chunks, vx_chunk_builder**
   vx_chunk_builder* b = new (std::nothrow) vx_chunk_builder();
   if (!b) return vx_fail(VX_E_NOMEM, "vx_chunk_builder_create: host allocation");
   b->ctx = ctx;
+  vx_internal_child(ctx, +1);  // released by vx_chunk_builder_destroy, also on the failure path below
   b->max_chunks = max_chunks;
   b->host_counts.assign(2 * (size_t)max_chunks, 0u);
   Guard g(ar.device);
@@ -473,6 +474,7 @@ void vx_chunk_builder_destroy(vx_chunk_builder* b) {
   for (void* p : {(void*)b->ws, (void*)b->counts, (void*)b->index, (void*)b->oy, (void*)b->node_base, (void*)b->macro_base,
                   (void*)b->colinfo_base, (void*)b->err})
     if (p) hipFree(p);
+  vx_internal_child(b->ctx, -1);
   delete b;
 }
 

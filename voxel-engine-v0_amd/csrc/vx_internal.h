@@ -43,6 +43,10 @@ struct vx_internal_arenas {
 void vx_internal_get_arenas(const vx_ctx* ctx, vx_internal_arenas* out);
 // Marks the node arena as changed (per-slot derived flags are refreshed before the next render).
 void vx_internal_nodes_changed(vx_ctx* ctx);
+// Raises (+1) or lowers (-1) the context's count of live child objects
+// (views, batches, servers, chunk builders); vx_destroy refuses while it is
+// above zero.
+void vx_internal_child(vx_ctx* ctx, int delta);
 
 // The device chunk builder over chunks at any positions of resident material
 // and ground arrays: batch chunk i reads position index[i] (host array; null:

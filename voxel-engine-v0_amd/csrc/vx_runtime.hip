@@ -64,6 +64,10 @@ struct vx_ctx {
   uint32_t* qcursor = nullptr;
   vx_query_config qcfg{8u, 48u, 0u, 0u};
   hipEvent_t qev[2] = {nullptr, nullptr};
+  // Live objects created on this context (views, batches, servers, chunk
+  // builders).  vx_destroy refuses while any is alive: each of them launches
+  // on the context's stream and reads or writes its arenas.
+  std::atomic<int> children{0};
 };
 
 uint64_t vx_internal_node_capacity(const vx_ctx* ctx) { return ctx->node_cap; }
@@ -79,6 +83,7 @@ void vx_internal_get_arenas(const vx_ctx* ctx, vx_internal_arenas* out) {
   out->device = ctx->device;
 }
 void vx_internal_nodes_changed(vx_ctx* ctx) { ctx->lights_dirty = true; }
+void vx_internal_child(vx_ctx* ctx, int delta) { ctx->children.fetch_add(delta); }
 
 struct vx_view {
   vx_ctx* ctx = nullptr;
@@ -292,8 +297,12 @@ int vx_create(const vx_caps* caps, vx_ctx** out) {
   return VX_OK;
 }
 
-void vx_destroy(vx_ctx* ctx) {
-  if (!ctx) return;
+int vx_destroy(vx_ctx* ctx) {
+  if (!ctx) return VX_OK;
+  const int live = ctx->children.load();
+  if (live > 0)
+    return vx_fail(VX_E_BUSY, "vx_destroy: %d view/batch/server/chunk-builder object(s) of this context are still alive; "
+                   "destroy them first", live);
   DeviceGuard g(ctx->device);
   if (ctx->stream) {
     hipStreamSynchronize(ctx->stream);
@@ -314,6 +323,7 @@ void vx_destroy(vx_ctx* ctx) {
   for (auto& e : ctx->qev)
     if (e) hipEventDestroy(e);
   delete ctx;
+  return VX_OK;
 }
 
 int vx_upload_regions(vx_ctx* ctx, int arena, const vx_region* regions, uint32_t n) {
@@ -447,6 +457,7 @@ int vx_view_create(vx_ctx* ctx, uint32_t pw, uint32_t ph, uint32_t rw, uint32_t 
   vx_view* v = new (std::nothrow) vx_view();
   if (!v) return vx_fail(VX_E_NOMEM, "vx_view_create: host allocation");
   v->ctx = ctx;
+  ctx->children.fetch_add(1);  // released by vx_view_destroy, also on the failure path below
   v->pw = pw;
   v->ph = ph;
   v->rw = rw;
@@ -475,6 +486,7 @@ void vx_view_destroy(vx_view* v) {
   dfree(v->counters);
   for (int i = 0; i < 10; ++i)
     if (v->ev[i]) hipEventDestroy(v->ev[i]);
+  v->ctx->children.fetch_sub(1);
   delete v;
 }
 
@@ -838,6 +850,7 @@ int vx_batch_create(vx_ctx* ctx, vx_view* const* views, uint32_t n, vx_batch** o
   vx_batch* b = new (std::nothrow) vx_batch();
   if (!b) return vx_fail(VX_E_NOMEM, "vx_batch_create: host allocation");
   b->ctx = ctx;
+  ctx->children.fetch_add(1);  // released by vx_batch_destroy
   rc = dalloc(&b->cursor, 2u * vxd::kBatchBands);
   for (int i = 0; i < 2 && !rc; ++i)
     if (hipEventCreate(&b->ev[i]) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "vx_batch_create: hipEventCreate");
@@ -860,6 +873,7 @@ void vx_batch_destroy(vx_batch* b) {
   dfree(b->cursor);
   for (auto& e : b->ev)
     if (e) hipEventDestroy(e);
+  b->ctx->children.fetch_sub(1);
   delete b;
 }
 

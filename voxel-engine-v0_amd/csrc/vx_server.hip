@@ -155,6 +155,11 @@ struct vx_server {
   uint32_t building = 0;             // builds taken by workers and not yet in `ready`
   std::deque<uint32_t> rebuild_q;
   std::vector<ChunkBuild> ready;     // rebuilt, waiting for a launch boundary
+  // Rebuilt chunks the node arena could not place at the last boundary.  They
+  // are retried at the next boundary that something else starts; the device
+  // thread does not wake for them alone (the arena changes only when other
+  // builds are applied, so retrying on their own would spin).
+  std::vector<ChunkBuild> deferred;
   std::vector<std::thread> workers;
   std::condition_variable wcv;
   // requests
@@ -188,6 +193,7 @@ struct vx_server {
   uint32_t build_batch = 0;
   std::unordered_map<uint32_t, std::unordered_map<uint32_t, uint8_t>> pend;
   std::deque<uint32_t> dev_q;
+  std::vector<uint32_t> dev_deferred;  // device builds the arena could not place (as `deferred`)
   bool dev_busy = false;
   uint64_t* d_off = nullptr;
   uint8_t* d_val = nullptr;
@@ -241,9 +247,12 @@ void worker_main(vx_server* s) {
                           VX_MAX_CHUNK_NODES, &count, b.macro, b.colinfo);
     }
     if (rc) {
-      std::lock_guard<std::mutex> lk(s->mu);
-      --s->building;
-      s->last_error = rc;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        --s->building;
+        s->last_error = rc;
+      }
+      s->cv.notify_all();  // vx_server_wait_edits and vx_server_drain return on the error
       continue;
     }
     ChunkBuild out;
@@ -277,6 +286,7 @@ int apply_ready(vx_server* s, std::vector<ChunkBuild>& builds) {
   std::vector<vx_region> rn, rr, rm, rc, mr;
   std::vector<std::pair<uint64_t, uint64_t>> old;
   std::vector<std::pair<uint32_t, vx_chunk_meta>> swaps;
+  std::vector<std::pair<uint32_t, uint64_t>> placed;  // (slot, generation) of the builds that get a range
   std::vector<ChunkBuild> later;
   for (ChunkBuild& b : builds) {
     uint64_t off = 0;
@@ -292,6 +302,7 @@ int apply_ready(vx_server* s, std::vector<ChunkBuild>& builds) {
     for (uint32_t w : b.macro) any |= w != 0u;
     m.macro_empty = any ? 0u : 1u;
     swaps.emplace_back(b.slot, m);
+    placed.emplace_back(b.slot, b.gen);
     rn.push_back(vx_region{off, b.nodes.size(), b.nodes.data()});
     rr.push_back(vx_region{off, b.ropes.size(), b.ropes.data()});
     rm.push_back(vx_region{m.macro_base, 16, b.macro});
@@ -310,15 +321,25 @@ int apply_ready(vx_server* s, std::vector<ChunkBuild>& builds) {
       s->meta[sw.first] = sw.second;
       mr.push_back(vx_region{sw.first, 1, &s->meta[sw.first]});
     }
-    for (ChunkBuild& b : builds) s->applied[b.slot] = std::max(s->applied[b.slot], b.gen);
     if ((rc0 = vx_upload_regions(s->ctx, VX_ARENA_META, mr.data(), (uint32_t)mr.size()))) return rc0;
     for (auto& o : old) s->arena.release(o.first, o.second);
     std::lock_guard<std::mutex> lk(s->mu);
+    for (auto& pg : placed) s->applied[pg.first] = std::max(s->applied[pg.first], pg.second);
     s->st.edits_applied += swaps.size();
   }
   if (!later.empty()) {
+    // Old ranges released above may make room now: then retry at once (the
+    // device thread wakes for `ready`); otherwise wait for a boundary that
+    // applies other builds.
+    const uint64_t free_total = s->arena.free_total();
+    const bool arena_changed = !swaps.empty();
     std::lock_guard<std::mutex> lk(s->mu);
-    for (auto& b : later) s->ready.push_back(std::move(b));
+    for (auto& b : later) {
+      if (b.nodes.size() > free_total && s->last_error == VX_OK)
+        s->last_error = vx_fail(VX_E_CAPACITY, "server: chunk slot %u needs %zu nodes, %llu free in the node arena",
+                                b.slot, b.nodes.size(), (unsigned long long)free_total);
+      (arena_changed ? s->ready : s->deferred).push_back(std::move(b));
+    }
   }
   return VX_OK;
 }
@@ -374,6 +395,13 @@ int apply_device_builds(vx_server* s, const std::vector<uint32_t>& slots,
   for (uint32_t i = 0; i < n; ++i) {
     if (!s->arena.alloc(counts[i], base[i])) {
       for (uint32_t j = 0; j < i; ++j) s->arena.release(base[j], counts[j]);
+      const uint64_t free_total = s->arena.free_total();
+      if (counts[i] > free_total) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->last_error == VX_OK)
+          s->last_error = vx_fail(VX_E_CAPACITY, "server: chunk slot %u needs %u nodes, %llu free in the node arena",
+                                  slots[i], counts[i], (unsigned long long)free_total);
+      }
       requeue = true;
       return VX_OK;
     }
@@ -434,6 +462,13 @@ double pct(std::vector<double> v, double p) {
   return v[i];
 }
 
+// The camera fields a ray query reads: voxel size, chunk size and count, grid origin and dims.
+bool same_scene(const vx_camera& a, const vx_camera& b) {
+  return a.voxel_params[0] == b.voxel_params[0] && a.chunk_size == b.chunk_size && a.chunk_count == b.chunk_count &&
+         std::memcmp(a.grid_origin_chunk, b.grid_origin_chunk, 3 * sizeof(int32_t)) == 0 &&
+         std::memcmp(a.grid_dims, b.grid_dims, 3 * sizeof(uint32_t)) == 0;
+}
+
 uint64_t request_rays(const Pending& p) {
   return p.req.kind == VX_REQ_VIEW ? (uint64_t)p.req.width * p.req.height : (uint64_t)p.rays.size();
 }
@@ -489,6 +524,10 @@ void device_main(vx_server* s) {
       s->cv.wait(lk, [&] { return s->stop || !s->inbox.empty() || !s->ready.empty() || !s->dev_q.empty(); });
       if (s->stop) break;
       builds.swap(s->ready);
+      for (ChunkBuild& b : s->deferred) builds.push_back(std::move(b));
+      s->deferred.clear();
+      s->dev_q.insert(s->dev_q.begin(), s->dev_deferred.begin(), s->dev_deferred.end());  // oldest first
+      s->dev_deferred.clear();
       // Device builds: the oldest dirty slots, up to one builder batch, with their pending edits.
       while (!s->dev_q.empty() && dev_slots.size() < s->build_batch) {
         const uint32_t slot = s->dev_q.front();
@@ -547,7 +586,7 @@ void device_main(vx_server* s) {
         for (uint32_t slot : dev_slots)
           if (!s->dirty[slot]) {
             s->dirty[slot] = 1;
-            s->dev_q.push_back(slot);
+            s->dev_deferred.push_back(slot);
           }
       } else if (!rc) {
         for (size_t i = 0; i < dev_slots.size(); ++i)
@@ -593,7 +632,15 @@ void device_main(vx_server* s) {
       }
       if (!rc && hipMemcpyAsync(d_rays, h_rays, nr * sizeof(vx_ray), hipMemcpyHostToDevice, s->qstream) != hipSuccess)
         rc = vx_fail(VX_E_DEVICE, "server: ray upload");
-      if (!rc) rc = vx_query_rays(s->ctx, &rays[0].req.camera, d_rays, (uint32_t)nr, d_hits, s->qstream);
+      // One fused launch per run of consecutive requests that share the scene
+      // fields of their camera (normally a single launch): a request is
+      // traced against the scene its own camera names.
+      for (size_t i = 0, r0 = 0; i < rays.size() && !rc; r0 += rays[i].rays.size(), ++i) {
+        size_t j = i, n_run = rays[i].rays.size();
+        while (j + 1 < rays.size() && same_scene(rays[j + 1].req.camera, rays[i].req.camera)) n_run += rays[++j].rays.size();
+        if (n_run) rc = vx_query_rays(s->ctx, &rays[i].req.camera, d_rays + r0, (uint32_t)n_run, d_hits + r0, s->qstream);
+        for (; i < j; ++i) r0 += rays[i].rays.size();
+      }
       // Hit downloads queue behind the query on its stream (only the ranges of
       // requests that asked for their hits), overlapping the view passes.
       size_t off = 0;
@@ -673,6 +720,7 @@ int vx_server_create(vx_ctx* ctx, const vx_host_world_view* world, const vx_serv
   vx_server* s = new (std::nothrow) vx_server();
   if (!s) return vx_fail(VX_E_NOMEM, "vx_server_create: host allocation");
   s->ctx = ctx;
+  vx_internal_child(ctx, +1);  // released by vx_server_destroy
   s->cfg = *cfg;
   if (!s->cfg.max_views_per_launch) s->cfg.max_views_per_launch = 64;
   if (!s->cfg.edit_threads) s->cfg.edit_threads = 8;
@@ -688,6 +736,7 @@ int vx_server_create(vx_ctx* ctx, const vx_host_world_view* world, const vx_serv
   s->arena.init(vx_internal_node_capacity(ctx), used);  // the node arena is the context's node capacity
   if (hipSetDevice(vx_internal_device(ctx)) != hipSuccess ||
       hipStreamCreateWithFlags(&s->qstream, hipStreamNonBlocking) != hipSuccess) {
+    vx_internal_child(ctx, -1);
     delete s;
     return vx_fail(VX_E_DEVICE, "vx_server_create: query stream");
   }
@@ -731,6 +780,7 @@ void vx_server_destroy(vx_server* s) {
   for (void* p : {(void*)s->dmat, (void*)s->dground, (void*)s->d_off, (void*)s->d_val})
     if (p) hipFree(p);
   if (s->qstream) hipStreamDestroy(s->qstream);
+  vx_internal_child(s->ctx, -1);
   delete s;
 }
 
@@ -875,20 +925,32 @@ int vx_server_close_stream(vx_server* s, uint32_t stream_id) {
 int vx_server_drain(vx_server* s) {
   if (!s) return vx_fail(VX_E_INVALID, "vx_server_drain: null server");
   std::unique_lock<std::mutex> lk(s->mu);
-  s->cv.wait(lk, [&] { return s->inbox.empty() && s->st.completed == s->st.admitted; });
+  s->cv.wait(lk, [&] { return s->last_error != VX_OK || (s->inbox.empty() && s->st.completed == s->st.admitted); });
   return s->last_error;
 }
 
 int vx_server_wait_edits(vx_server* s) {
   if (!s) return vx_fail(VX_E_INVALID, "vx_server_wait_edits: null server");
   std::unique_lock<std::mutex> lk(s->mu);
-  // every queued edit has been built and applied at a launch boundary
+  // Every queued edit has been built and applied at a launch boundary, or an
+  // error stopped one of them (a failed fill, build or upload), or only
+  // builds the node arena cannot place are left: nothing else is queued or
+  // running, so no boundary will free room for them.
+  bool stuck = false;
   s->cv.wait(lk, [&] {
+    if (s->last_error != VX_OK) return true;
     if (!s->rebuild_q.empty() || !s->ready.empty() || s->building || !s->dev_q.empty() || s->dev_busy) return false;
+    if (!s->deferred.empty() || !s->dev_deferred.empty()) {
+      if (!s->inbox.empty()) return false;  // the next launch boundary retries them
+      stuck = true;
+      return true;
+    }
     for (size_t i = 0; i < s->gen.size(); ++i)
       if (s->applied[i] < s->gen[i]) return false;
     return true;
   });
+  if (s->last_error == VX_OK && stuck)
+    return vx_fail(VX_E_CAPACITY, "vx_server_wait_edits: rebuilt chunks do not fit the node arena");
   return s->last_error;
 }
 

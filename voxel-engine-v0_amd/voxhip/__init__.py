@@ -193,7 +193,7 @@ def _load() -> C.CDLL:
     vp, u32, i32, u64, f32 = C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_float
     sig = {
         "vx_last_error": ([], C.c_char_p), "vx_version": ([], C.c_char_p), "vx_device_count": ([], C.c_int),
-        "vx_create": ([C.POINTER(Caps), C.POINTER(vp)], C.c_int), "vx_destroy": ([vp], None),
+        "vx_create": ([C.POINTER(Caps), C.POINTER(vp)], C.c_int), "vx_destroy": ([vp], C.c_int),
         "vx_upload_regions": ([vp, C.c_int, C.POINTER(Region), u32], C.c_int),
         "vx_write_grid": ([vp, vp, u32], C.c_int),
         "vx_write_clipmap": ([vp, C.POINTER(Clipmap), C.POINTER(ClipPatch), u32], C.c_int),
@@ -558,8 +558,11 @@ class Context:
         _check(LIB.vx_sync(self._h), "vx_sync")
 
     def close(self) -> None:
+        """Frees the resident world.  The library refuses (VX_E_BUSY) while a
+        view, batch, server or chunk builder of this context is alive; the
+        context then stays open and the error is raised."""
         if self._h:
-            LIB.vx_destroy(self._h)
+            _check(LIB.vx_destroy(self._h), "vx_destroy")
             self._h = None
 
     def __del__(self):
