@@ -387,10 +387,7 @@ def batching(vx, scenes, sc, hw, ctx, n_views, steps):
     world, each step one full frame of every view.  Plain = one view per
     launch sequence (vx_render per view); batched = one fused launch per pass
     (vx_batch_render), after the autotuner picked its configuration."""
-    rng = np.random.default_rng(7)
-    sizes = [(320, 180), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
-    res = [sizes[i] for i in rng.integers(0, len(sizes), n_views)]
-    frames = rng.integers(0, len(sc.poses), n_views)
+    res, frames = scenes.batch_views(n_views, len(sc.poses))
     rays = int(sum(w * h for w, h in res))
 
     def cams(step):
@@ -448,13 +445,10 @@ def mixed(vx, scenes, n_views=64, n_batches=256, steps=2):
     all on the context stream.  Batched: the views as one fused batch
     (vx_batch_render, autotuned) on the context stream while all ray batches
     run as one fused query launch (autotuned) on a second stream."""
-    rng_v = np.random.default_rng(7)
-    sizes_v = [(320, 180), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
-    res = [sizes_v[i] for i in rng_v.integers(0, len(sizes_v), n_views)]
-    rng_q = np.random.default_rng(11)
-    sizes_q = np.exp(rng_q.uniform(np.log(1e3), np.log(1e6), n_batches)).astype(np.int64)
-    n_rays = int(sizes_q.sum())
     sc = scenes.c4(streams=n_views, frames=200)
+    res, _ = scenes.batch_views(n_views, len(sc.poses))
+    sizes_q, rays = scenes.query_batches(n_batches, sc.grid_origin, sc.grid_dims)
+    n_rays = int(sizes_q.sum())
     hw = vx.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
     clip, heights = scenes.scene_clipmap(sc, 0)
     ctx = vx.Context(node_capacity=len(hw.nodes), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
@@ -463,7 +457,6 @@ def mixed(vx, scenes, n_views=64, n_batches=256, steps=2):
     ctx.write_clipmap(clip, heights)
     qcam = vx.camera_build(vx.make_pose((1.0, 1.0, 1.0), 0.0, 0.0), (64, 64), (64, 64), 0, grid_origin=sc.grid_origin,
                            grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
-    rays = scenes.query_rays(rng_q, n_rays, sc.grid_origin, sc.grid_dims, tmax_m=51.2)
     d_rays = vx.DeviceArray.from_numpy(rays)
     d_hits = vx.DeviceArray(n_rays * vx.HIT_DTYPE.itemsize)
     offs = np.concatenate([[0], np.cumsum(sizes_q)])
@@ -543,10 +536,9 @@ def ray_queries(vx, scenes, n_batches):
     world's AABB, directions uniform on the sphere, max distance 512 voxels.
     Fused = all batches in one vx_query_rays launch (after the autotuner);
     plain = one launch per batch."""
-    rng = np.random.default_rng(11)
-    sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n_batches)).astype(np.int64)
-    n = int(sizes.sum())
     dims, origin = (32, 4, 32), (0, 0, 0)
+    sizes, rays = scenes.query_batches(n_batches, origin, dims)
+    n = int(sizes.sum())
     t0 = time.time()
     hw = vx.HostWorld(vx.world_params(42), origin, dims)
     build_s = time.time() - t0
@@ -555,7 +547,6 @@ def ray_queries(vx, scenes, n_batches):
     ctx.upload_world(hw)
     cam = vx.camera_build(vx.make_pose((1.0, 1.0, 1.0), 0.0, 0.0), (64, 64), (64, 64), 0, grid_origin=origin,
                           grid_dims=dims, chunk_count=hw.chunk_count)
-    rays = scenes.query_rays(rng, n, origin, dims, tmax_m=51.2)
     d_rays = vx.DeviceArray.from_numpy(rays)
     d_hits = vx.DeviceArray(n * vx.HIT_DTYPE.itemsize)
     tuned = vx.query_tune(ctx, cam, d_rays, d_hits, n).as_dict()

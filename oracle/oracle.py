@@ -73,6 +73,7 @@ def _load() -> C.CDLL:
     lib.vxo_intersect_aabb.argtypes = [vp, vp, vp, vp, vp]
     lib.vxo_fog_od.argtypes = [f32, vp, vp, f32]
     lib.vxo_fog_od.restype = f32
+    lib.vxo_tile_raw_counts.argtypes = [C.POINTER(Scene), vp, u32, u32, vp]
     return lib
 
 
@@ -139,6 +140,14 @@ def query_rays(world: World, cam, rays: np.ndarray, hits_dtype, threads: int = 0
     assert LIB.vxo_query_rays(C.byref(world.scene), C.addressof(cam), rays.ctypes.data, len(rays), hits.ctypes.data,
                               threads, C.byref(counts) if counts is not None else None) == 0
     return hits
+
+
+def tile_raw_counts(world: World, cam, w: int, h: int) -> np.ndarray:
+    """Per 8x8 tile (row-major), the raw candidate-chunk count of the tile
+    gather (before the 192 cap and the 24-entry truncation)."""
+    out = np.zeros(((h + 7) // 8, (w + 7) // 8), np.uint32)
+    assert LIB.vxo_tile_raw_counts(C.byref(world.scene), C.addressof(cam), w, h, out.ctypes.data) == 0
+    return out
 
 
 def local_taa(fr: Frame):

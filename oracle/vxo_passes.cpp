@@ -1304,4 +1304,30 @@ float vxo_fog_od(float base, const float* ro, const float* rd, float t) {
   return vxo::fog_optical_depth_with_base(base, vxo::V3(ro[0], ro[1], ro[2]), vxo::V3(rd[0], rd[1], rd[2]), t);
 }
 
+// The tile candidate gather of main_primary (ray_main.wgsl:179-197) alone:
+// per 8x8 tile, the raw candidate count the two probe rays produce (the
+// atomic counter value, which may exceed MAX_TILE_CHUNKS) — for tests that
+// must know whether a frame reaches the 24-candidate truncation.
+int vxo_tile_raw_counts(const vxo_scene* scene, const vx_camera* cam, uint32_t w, uint32_t h, uint32_t* raw) {
+  if (!scene || !cam || !w || !h || !raw) return -1;
+  const uint32_t tx = (w + 7) / 8, ty = (h + 7) / 8;
+  vxo::Counts cnt;
+  std::memset(&cnt, 0, sizeof(cnt));
+  for (uint32_t t = 0; t < tx * ty; ++t) {
+    vxo::Ctx C = make_ctx(scene, cam, &cnt, false);
+    vxo::TileList L;
+    L.count = 0;
+    if (cam->chunk_count != 0u) {
+      const uint32_t wgx = t % tx, wgy = t / tx;
+      vxo::v2 base = vxo::V2((float)(wgx * vxo::TILE_SIZE), (float)(wgy * vxo::TILE_SIZE));
+      vxo::tile_append_candidates_for_ray(C, L, vxo::cam_pos(C), vxo::ray_dir_from_pixel(C, base + vxo::V2(4.5f, 4.5f)),
+                                          0.0f, vxo::FOG_MAX_DIST);
+      vxo::tile_append_candidates_for_ray(C, L, vxo::cam_pos(C), vxo::ray_dir_from_pixel(C, base + vxo::V2(1.5f, 1.5f)),
+                                          0.0f, vxo::FOG_MAX_DIST);
+    }
+    raw[t] = L.count;
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -150,3 +150,26 @@ def c4(seed: int = 42, streams: int = 32, frames: int = 600) -> Scene:
     s.poses = flight_poses(p, dims, seed, frames)
     s.stream_poses = [flight_poses(p, dims, seed + 1 + i, frames) for i in range(streams)]
     return s
+
+
+VIEW_SIZES = [(320, 180), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
+
+
+def batch_views(n_views: int, n_poses: int, seed: int = 7):
+    """The multi-view batching workload's views: resolutions drawn uniformly
+    from VIEW_SIZES and starting frames of the scene's flight, both seeded.
+    Returns (resolutions, frames)."""
+    rng = np.random.default_rng(seed)
+    res = [VIEW_SIZES[i] for i in rng.integers(0, len(VIEW_SIZES), n_views)]
+    frames = rng.integers(0, n_poses, n_views)
+    return res, frames
+
+
+def query_batches(n_batches: int, grid_origin, grid_dims, seed: int = 11, lo: float = 1e3, hi: float = 1e6):
+    """The incoherent ray-query workload: n_batches batch sizes log-uniform in
+    [lo, hi] and all their rays (query_rays) from one seeded generator.
+    Returns (sizes, rays)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_batches)).astype(np.int64)
+    rays = query_rays(rng, int(sizes.sum()), grid_origin, grid_dims, tmax_m=51.2)
+    return sizes, rays
