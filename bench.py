@@ -40,7 +40,6 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--res", default="3840x2160")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--staged", action="store_true", help="primary pass as staged kernels over pixel queues")
     ap.add_argument("--batch-views", type=int, default=64,
                     help="views of the mixed-resolution batching workload (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -149,7 +148,7 @@ def main():
     hw = vx.HostWorld(sc.params, sc.grid_origin, sc.grid_dims)
     clip, heights = scenes.scene_clipmap(sc, 0)
     ctx = vx.Context(node_capacity=len(hw.nodes), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
-                     clip_levels=5, clip_res=256, staged=a.staged)
+                     clip_levels=5, clip_res=256)
     ctx.upload_world(hw)
     ctx.write_clipmap(clip, heights)
     view = vx.View(ctx, (W, H), (W, H))
@@ -291,7 +290,7 @@ def main():
                                f"5-level clipmap ({'static' if a.static_clipmap else 'updated every frame on the device'}), "
                                f"camera flight frames {a.warmup}..{a.warmup + a.steps - 1}",
                    "resolution": [W, H], "chunks": hw.chunk_count, "nodes": int(len(hw.nodes)),
-                   "path": "plain", "primary_kernels": "staged" if a.staged else "persistent work-stealing over the view's 8x8 tiles"},
+                   "path": "plain", "primary_kernels": "persistent work-stealing over the view's 8x8 tiles"},
         "primary_kernel_rays_per_s": round(rays / (prim_ms * 1e-3), 1),
         "pass_ms_mean": {n: round(float(v), 4) for n, v in
                          zip(("primary", "local_taa", "godray", "composite", "composite_taa"), pass_ms.mean(axis=0))},

@@ -160,9 +160,6 @@ typedef struct vx_caps {
  * shader declares for macro_occ (render/state/bindgroups.rs:142-151 vs
  * common.wgsl:407-409).  VX_CTX_GODRAY_MACRO_FIX makes it read the macro words. */
 #define VX_CTX_GODRAY_MACRO_FIX 1u
-/* Run the primary pass as staged kernels over per-class pixel queues instead of
- * one kernel per 8x8 tile (same results; a scheduling choice). */
-#define VX_CTX_PRIMARY_STAGED   2u
 
 /* Renderer::new (render/state/mod.rs:147-268): allocates the resident world.
  * vx_destroy frees it (the Renderer's drop).  Teardown order: every object

@@ -110,9 +110,8 @@ void fill_taper(float* taper) {
 
 extern "C" {
 
-// mode 0: the one-kernel pass (k_primary<false>); 1: the staged kernels over
-// pixel queues; 2: the load-counting kernel (k_primary<true>), which also adds
-// its counts into counts[9].
+// mode 0: the one-kernel pass (k_primary<false>); 2: the load-counting kernel
+// (k_primary<true>), which also adds its counts into counts[9].
 int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, uint32_t h, const float* primary_hist_in,
                     const float* shadow_in, float* color, float* depth, float* local, float* primary_hist_out,
                     float* shadow_out, vx_hit* hits, int mode, int threads, uint64_t* counts) {
@@ -163,93 +162,6 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
           if (gx < w && gy < h) counted::primary_pixel(C, L, fb, gx, gy, counted::sky_bg(F3(0.0f, 1.0f, 0.0f)));
         }
       }
-    });
-  } else if (mode == 1) {
-    // Staged: trace per tile into per-class queues, then each stage over its queue.
-    const size_t n = (size_t)w * h;
-    std::vector<PrimRec> rec(n);
-    std::vector<std::vector<uint32_t>> tq[kNumPrimQueues];
-    for (auto& q : tq) q.resize(nt);
-    float taper[12];
-    fill_taper(taper);
-    parallel_for(nt, threads, [&](uint32_t tile, int) {
-      TileLDS L;
-      fill_taper(L.grass_taper);
-      const uint32_t wgx = tile % tx, wgy = tile / tx;
-      plain::Ctx C;
-      C.S = S;
-      C.cam = cam;
-      C.grass_taper = L.grass_taper;
-      plain::tile_build(C, L, wgx, wgy);
-      for (uint32_t lane = 0; lane < 64u; ++lane) {
-        const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
-        C.grass_blades = L.grass_blades + lane;
-        if (gx >= w || gy >= h) continue;
-        const plain::PrimRay r = plain::prim_ray(C, fb, gx, gy);
-        const plain::VoxTraceResult vt = plain::prim_trace(C, L, fb, r);
-        const uint32_t pix = (uint32_t)r.pix;
-        if (plain::prim_is_voxel(vt)) {
-          tq[kQVox][tile].push_back(pix);
-          if (vt.best.mat == MAT_GRASS) tq[kQGrass][tile].push_back(pix);
-          if (plain::prim_shadow_pixel(r)) tq[kQShadow][tile].push_back(pix);
-          plain::prim_rec_store(rec.data(), pix, vt.best, plain::prim_anchor(vt));
-        } else {
-          tq[kQEnv][tile].push_back(pix);
-        }
-      }
-    });
-    std::vector<uint32_t> q[kNumPrimQueues];
-    for (uint32_t k = 0; k < kNumPrimQueues; ++k)
-      for (auto& part : tq[k]) q[k].insert(q[k].end(), part.begin(), part.end());
-    std::vector<float> blade_cache((size_t)threads * kGrassBladeCache);
-    auto ctx = [&](int t) {
-      plain::Ctx C;
-      C.S = S;
-      C.cam = cam;
-      C.grass_taper = taper;
-      C.grass_blades = blade_cache.data() + (size_t)t * kGrassBladeCache;
-      return C;
-    };
-    auto ray = [&](const plain::Ctx& C, uint32_t pix) { return plain::prim_ray(C, fb, pix % w, pix / w); };
-    parallel_for((uint32_t)q[kQGrass].size(), threads, [&](uint32_t i, int t) {
-      const plain::Ctx C = ctx(t);
-      const uint32_t pix = q[kQGrass][i];
-      plain::PrimAnchor a;
-      plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
-      plain::prim_grass_refine(C, ray(C, pix), hg);
-      plain::prim_rec_store_tn(rec.data(), pix, hg);
-    });
-    parallel_for((uint32_t)q[kQShadow].size(), threads, [&](uint32_t i, int t) {
-      const plain::Ctx C = ctx(t);
-      const uint32_t pix = q[kQShadow][i];
-      plain::PrimAnchor a;
-      const plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
-      fb.shadow_out[pix] = plain::prim_shadow(C, fb, ray(C, pix), hg);
-    });
-    std::vector<uint8_t> lit(n, 0);
-    parallel_for((uint32_t)q[kQVox].size(), threads, [&](uint32_t i, int t) {
-      const plain::Ctx C = ctx(t);
-      const uint32_t pix = q[kQVox][i];
-      const plain::PrimRay r = ray(C, pix);
-      plain::PrimAnchor a;
-      const plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
-      const float shadow_store = plain::prim_shadow_pixel(r) ? fb.shadow_out[pix] : plain::prim_shadow(C, fb, r, hg);
-      lit[pix] = plain::prim_shade_voxel(C, fb, r, hg, shadow_store, a, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
-      if (!lit[pix]) fb.local[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    });
-    for (uint32_t pix : q[kQVox])
-      if (lit[pix]) q[kQLight].push_back(pix);
-    parallel_for((uint32_t)q[kQLight].size(), threads, [&](uint32_t i, int t) {
-      const plain::Ctx C = ctx(t);
-      const uint32_t pix = q[kQLight][i];
-      plain::PrimAnchor a;
-      const plain::HitGeom hg = plain::prim_rec_load(rec.data(), pix, a);
-      fb.local[pix] = plain::prim_local_light(C, ray(C, pix), hg);
-    });
-    parallel_for((uint32_t)q[kQEnv].size(), threads, [&](uint32_t i, int t) {
-      const plain::Ctx C = ctx(t);
-      const uint32_t pix = q[kQEnv][i];
-      plain::prim_env(C, fb, ray(C, pix), plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
     });
   } else {
     return -1;

@@ -12,9 +12,9 @@ DEPTH_REL_TOL = 1e-5
 COLOR_ABS_TOL = 1e-3
 
 
-def make_context(vx, hw, clip_levels=5, godray_macro_fix=False, staged=False):
+def make_context(vx, hw, clip_levels=5, godray_macro_fix=False):
     ctx = vx.Context(node_capacity=max(len(hw.nodes), 1), chunk_capacity=hw.chunk_count, grid_capacity=len(hw.grid),
-                     clip_levels=clip_levels, clip_res=256, godray_macro_fix=godray_macro_fix, staged=staged)
+                     clip_levels=clip_levels, clip_res=256, godray_macro_fix=godray_macro_fix)
     ctx.upload_world(hw)
     return ctx
 

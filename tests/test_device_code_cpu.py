@@ -1,7 +1,6 @@
 """The product's device pass code (voxel-engine-v0_amd/csrc/vxd_passes.inc),
 compiled for the host by tests/cpu_kernels, against the oracle: the primary
-pass in its one-kernel form, its staged form (the stage functions run over
-per-class pixel queues) and its load-counting form.  Hit identity, colour,
+pass in its one-kernel form and its load-counting form.  Hit identity, colour,
 depth, history and shadow must be bit-identical, and the counting form must
 count exactly the loads the oracle counts.  Any loop that failed to end would
 hang here, on the CPU, before the code ever runs on a GPU."""

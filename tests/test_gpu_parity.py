@@ -33,8 +33,8 @@ def _assert_parity(s):
     assert s["color_max_abs"] <= COLOR_ABS_TOL, s
 
 
-@pytest.mark.parametrize("wind,staged", [(0.0, False), (2.0, False), (2.0, True)])
-def test_primary_c0_first_frame(gpu, wind, staged):
+@pytest.mark.parametrize("wind", [0.0, 2.0])
+def test_primary_c0_first_frame(gpu, wind):
     """Config 0: plain primary pass, 640x360, fresh history, fixed seeded pose."""
     O = _oracle()
     from voxhip import scenes
@@ -42,7 +42,7 @@ def test_primary_c0_first_frame(gpu, wind, staged):
     sc, hw = host_world("c0")
     clip, heights = scenes.scene_clipmap(sc)
     cam = scenes.frame_camera(sc, hw, 0, wind_strength=wind)
-    ctx = make_context(gpu, hw, staged=staged)
+    ctx = make_context(gpu, hw)
     ctx.write_clipmap(clip, heights)
     view = gpu.View(ctx, sc.present, sc.render)
     view.write_camera(cam)
@@ -62,7 +62,7 @@ def test_primary_c0_first_frame(gpu, wind, staged):
     s["hist_bit_exact"] = bool(np.array_equal(phist.view(np.uint32), fr.primary_hist.view(np.uint32)))
     s["shadow_max_abs"] = float(np.max(np.abs(shadow - fr.shadow)))
     s["kinds"] = np.bincount(fr.hits["kind"].ravel(), minlength=3).tolist()
-    _report(f"c0 primary wind={wind} staged={staged}", s)
+    _report(f"c0 primary wind={wind}", s)
     _assert_parity(s)
     assert s["local_max_abs"] <= COLOR_ABS_TOL
     assert s["shadow_max_abs"] <= COLOR_ABS_TOL
@@ -214,8 +214,7 @@ def test_godray_macro_binding_modes(gpu, fix):
     assert err <= COLOR_ABS_TOL
 
 
-@pytest.mark.parametrize("staged", [False, True])
-def test_flight_c1_world(gpu, staged):
+def test_flight_c1_world(gpu):
     """The bench world (16x4x16 chunks, 5-level clipmap) along the seeded
     camera flight, full pipeline, three frames at 640x360: the same world,
     grass density and camera path as the 4K benchmark, at a size the oracle
@@ -226,7 +225,7 @@ def test_flight_c1_world(gpu, staged):
     sc, hw = host_world("c1")
     clip, heights = scenes.scene_clipmap(sc, 0)
     res = (640, 360)
-    ctx = make_context(gpu, hw, staged=staged)
+    ctx = make_context(gpu, hw)
     ctx.write_clipmap(clip, heights)
     view = gpu.View(ctx, res, res)
     w = O.World.from_host(hw, clip, heights)
@@ -246,7 +245,7 @@ def test_flight_c1_world(gpu, staged):
         s["output_max_abs"] = float(np.max(np.abs(view.read(gpu.OUT_OUTPUT) - fr.output)))
         s["kinds"] = np.bincount(fr.hits["kind"].ravel(), minlength=3).tolist()
         s["grass_hits"] = int(np.sum((fr.hits["kind"] == 1) & (fr.hits["mat"] == gpu.MAT_GRASS)))
-        _report(f"c1 flight frame {frame} staged={staged}", s)
+        _report(f"c1 flight frame {frame}", s)
         _assert_parity(s)
         assert s["output_max_abs"] <= COLOR_ABS_TOL
 

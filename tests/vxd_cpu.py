@@ -1,7 +1,7 @@
 """The product's device pass code built for the host (tests/cpu_kernels):
 builds libvxd_cpu.so with g++ on first use and runs the primary pass in its
-one-kernel, staged and load-counting forms with host loops in place of the
-GPU's tiles, lanes and queues."""
+one-kernel and load-counting forms with host loops in place of the GPU's
+tiles and lanes."""
 from __future__ import annotations
 
 import ctypes as C
@@ -18,7 +18,7 @@ OUT = HERE / "cpu_kernels" / "_build" / "libvxd_cpu.so"
 CSRC = REPO / "voxel-engine-v0_amd" / "csrc"
 DEPS = [SRC, CSRC / "vxd_passes.inc", CSRC / "vxd_math.h", CSRC / "vxd_consts.h", CSRC / "vxd_scene.h",
         REPO / "include" / "voxhip.h", HERE / "cpu_kernels" / "shim" / "hip" / "hip_runtime.h"]
-MODES = {"one": 0, "staged": 1, "counted": 2}
+MODES = {"one": 0, "counted": 2}
 
 
 def build() -> Path:

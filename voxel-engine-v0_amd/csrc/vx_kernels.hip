@@ -114,171 +114,6 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary(
   }
 }
 
-// ---------------------------------------------------------------------------
-// Primary pass, product form: stages over pixel queues.
-//
-// Lanes of one wavefront diverge badly in the monolithic pass (sky next to
-// grass next to a 1-in-8 shadow ray next to a light gather), and its register
-// footprint spills.  The stages below each run one kind of work on a compact
-// queue of pixels, so a wavefront's lanes follow the same path and each
-// kernel keeps only its own state in registers:
-//   1. k_prim_trace   per 8x8 tile: candidate list, history window, voxel
-//                     trace; writes the voxel hit record and appends the
-//                     pixel to the queues it belongs to;
-//   2. k_prim_grass   voxel grass hits: grass blade refinement of t and n;
-//   3. k_prim_shadow  voxel pixels with (seed & 7) == 0: sun shadow;
-//   4. k_prim_shade   voxel pixels: shading, fog, stores; queues light gather;
-//   5. k_prim_light   pixels within the local-light range: light gather;
-//   6. k_prim_env     pixels without a voxel hit: heightfield or sky.
-// Queue order depends on wavefront scheduling; results do not (each item
-// reads and writes only its own pixel).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-
-// Wavefront-aggregated append: one atomic per wavefront and queue.  Must be
-// reached by all lanes of the wavefront.
-__device__ __forceinline__ void q_push(uint32_t* __restrict__ q, uint32_t* __restrict__ count, bool pred, uint32_t val) {
-  const unsigned long long mask = __ballot(pred);
-  if (mask == 0ull) return;
-  uint32_t base = 0u;
-  if (lane_id() == 0u) base = atomicAdd(count, (uint32_t)__popcll(mask));
-  base = __shfl(base, 0, 64);
-  if (pred) q[base + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull))] = val;
-}
-
-__global__ __launch_bounds__(64) void k_prim_reset(PrimQueues Q) {
-  if (threadIdx.x < kNumPrimQueues) Q.count[threadIdx.x] = 0u;
-}
-
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_prim_trace(
-    Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, PrimQueues Q, uint32_t tiles_x, uint32_t ntiles) {
-  __shared__ TileLDS L;
-  const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
-  const uint32_t wgx = tile % tiles_x, wgy = tile / tiles_x;
-  if (threadIdx.x < 12u) L.grass_taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
-  plain::Ctx C;
-  C.S = S;
-  C.cam = cam;
-  C.grass_taper = L.grass_taper;
-  C.grass_blades = L.grass_blades + threadIdx.x;
-  if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
-  __syncthreads();
-  const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
-  bool vox = false, grass = false, shadow = false, env = false;
-  uint32_t pix = 0u;
-  if (gx < fb.rw && gy < fb.rh) {
-    const plain::PrimRay r = plain::prim_ray(C, fb, gx, gy);
-    pix = (uint32_t)r.pix;
-    const plain::VoxTraceResult vt = plain::prim_trace(C, L, fb, r);
-    if (plain::prim_is_voxel(vt)) {
-      vox = true;
-      grass = vt.best.mat == MAT_GRASS;
-      shadow = plain::prim_shadow_pixel(r);
-      plain::prim_rec_store(Q.rec, pix, vt.best, plain::prim_anchor(vt));
-    } else {
-      env = true;
-    }
-  }
-  q_push(Q.q[kQVox], Q.count + kQVox, vox, pix);
-  q_push(Q.q[kQGrass], Q.count + kQGrass, grass, pix);
-  q_push(Q.q[kQShadow], Q.count + kQShadow, shadow, pix);
-  q_push(Q.q[kQEnv], Q.count + kQEnv, env, pix);
-}
-
-// Queue item of this lane (blocks dealt to XCDs in contiguous ranges), or
-// false past the queue's end.
-__device__ __forceinline__ bool q_item(const PrimQueues& Q, uint32_t which, uint32_t& pix) {
-  const uint32_t n = Q.count[which];
-  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * 64u + lane_id();
-  if (i >= n) return false;
-  pix = Q.q[which][i];
-  return true;
-}
-
-__device__ __forceinline__ plain::Ctx stage_ctx(const Scene& S, const vx_camera* cam) {
-  plain::Ctx C;
-  C.S = S;
-  C.cam = cam;
-  C.grass_taper = nullptr;
-  return C;
-}
-
-__global__ __launch_bounds__(64) void k_prim_grass(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, PrimQueues Q) {
-  __shared__ float taper[12];
-  __shared__ float blades[kGrassBladeCache];
-  if (threadIdx.x < 12u) taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
-  __syncthreads();
-  uint32_t pix;
-  if (!q_item(Q, kQGrass, pix)) return;
-  plain::Ctx C = stage_ctx(S, cam);
-  C.grass_taper = taper;
-  C.grass_blades = blades + threadIdx.x;
-  const plain::PrimRay r = plain::prim_ray(C, fb, pix % fb.rw, pix / fb.rw);
-  plain::PrimAnchor a;
-  plain::HitGeom h = plain::prim_rec_load(Q.rec, pix, a);
-  plain::prim_grass_refine(C, r, h);
-  plain::prim_rec_store_tn(Q.rec, pix, h);
-}
-
-__global__ __launch_bounds__(64) void k_prim_shadow(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, PrimQueues Q) {
-  uint32_t pix;
-  if (!q_item(Q, kQShadow, pix)) return;
-  const plain::Ctx C = stage_ctx(S, cam);
-  const plain::PrimRay r = plain::prim_ray(C, fb, pix % fb.rw, pix / fb.rw);
-  plain::PrimAnchor a;
-  const plain::HitGeom h = plain::prim_rec_load(Q.rec, pix, a);
-  fb.shadow_out[pix] = plain::prim_shadow(C, fb, r, h);
-}
-
-__global__ __launch_bounds__(64) void k_prim_shade(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, PrimQueues Q) {
-  __shared__ float taper[12];
-  __shared__ float blades[kGrassBladeCache];
-  if (threadIdx.x < 12u) taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
-  __syncthreads();
-  const uint32_t n = Q.count[kQVox];
-  const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * 64u + lane_id();
-  if (i - lane_id() >= n) return;  // whole wavefront past the end
-  bool lit = false;
-  uint32_t pix = 0u;
-  if (i < n) {
-    pix = Q.q[kQVox][i];
-    plain::Ctx C = stage_ctx(S, cam);
-    C.grass_taper = taper;
-    C.grass_blades = blades + threadIdx.x;
-    const plain::PrimRay r = plain::prim_ray(C, fb, pix % fb.rw, pix / fb.rw);
-    plain::PrimAnchor a;
-    const plain::HitGeom h = plain::prim_rec_load(Q.rec, pix, a);
-    const float shadow_store = plain::prim_shadow_pixel(r) ? fb.shadow_out[pix] : plain::prim_shadow(C, fb, r, h);
-    lit = plain::prim_shade_voxel(C, fb, r, h, shadow_store, a, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
-    if (!lit) fb.local[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  }
-  q_push(Q.q[kQLight], Q.count + kQLight, lit, pix);
-}
-
-__global__ __launch_bounds__(64) void k_prim_light(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, PrimQueues Q) {
-  uint32_t pix;
-  if (!q_item(Q, kQLight, pix)) return;
-  const plain::Ctx C = stage_ctx(S, cam);
-  const plain::PrimRay r = plain::prim_ray(C, fb, pix % fb.rw, pix / fb.rw);
-  plain::PrimAnchor a;
-  const plain::HitGeom h = plain::prim_rec_load(Q.rec, pix, a);
-  fb.local[pix] = plain::prim_local_light(C, r, h);
-}
-
-__global__ __launch_bounds__(64) void k_prim_env(Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, PrimQueues Q) {
-  __shared__ float taper[12];
-  __shared__ float blades[kGrassBladeCache];
-  if (threadIdx.x < 12u) taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
-  __syncthreads();
-  uint32_t pix;
-  if (!q_item(Q, kQEnv, pix)) return;
-  plain::Ctx C = stage_ctx(S, cam);
-  C.grass_taper = taper;
-  C.grass_blades = blades + threadIdx.x;
-  const plain::PrimRay r = plain::prim_ray(C, fb, pix % fb.rw, pix / fb.rw);
-  plain::prim_env(C, fb, r, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
-}
-
 __global__ __launch_bounds__(256) void k_local_taa(FrameBuffers fb) {
   const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u), y = blockIdx.y * 16u + (threadIdx.x >> 4);
   if (x < fb.rw && y < fb.rh) plain::local_taa_pixel(fb, x, y);
@@ -465,6 +300,8 @@ __global__ __launch_bounds__(256) void k_composite_taa_batch(Scene S, Batch B) {
 // wavefront does not idle behind its longest ray.  Workgroups exit when the
 // ray cursor has passed n and none of their lanes holds a ray.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
 __global__ __launch_bounds__(64) void k_query_begin(uint32_t* __restrict__ cursor) {
   if (threadIdx.x == 0) cursor[0] = 0u;
 }
@@ -522,19 +359,6 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_query(
 // Launchers (host)
 // ---------------------------------------------------------------------------
 
-hipError_t vx_launch_primary_staged(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb,
-                                    const vxd::PrimQueues& Q, hipStream_t st) {
-  const uint32_t tx = (fb.rw + 7u) / 8u, ty = (fb.rh + 7u) / 8u, nt = tx * ty;
-  const uint32_t items = (uint32_t)(((size_t)fb.rw * fb.rh + 63u) / 64u);  // queue length bound, in wavefronts
-  hipLaunchKernelGGL(k_prim_reset, dim3(1), dim3(64), 0, st, Q);
-  hipLaunchKernelGGL(k_prim_trace, dim3(nt), dim3(64), 0, st, S, cam_dev, fb, Q, tx, nt);
-  hipLaunchKernelGGL(k_prim_grass, dim3(items), dim3(64), 0, st, S, cam_dev, fb, Q);
-  hipLaunchKernelGGL(k_prim_shadow, dim3(items), dim3(64), 0, st, S, cam_dev, fb, Q);
-  hipLaunchKernelGGL(k_prim_shade, dim3(items), dim3(64), 0, st, S, cam_dev, fb, Q);
-  hipLaunchKernelGGL(k_prim_light, dim3(items), dim3(64), 0, st, S, cam_dev, fb, Q);
-  hipLaunchKernelGGL(k_prim_env, dim3(items), dim3(64), 0, st, S, cam_dev, fb, Q);
-  return hipGetLastError();
-}
 hipError_t vx_launch_primary(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
                              unsigned long long* counters, float* ray_bytes, hipStream_t st) {
   const uint32_t tx = (fb.rw + 7u) / 8u, ty = (fb.rh + 7u) / 8u, nt = tx * ty;

@@ -9,10 +9,6 @@
 // which also writes each ray's counted bytes to ray_bytes when it is not null).
 hipError_t vx_launch_primary(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
                              unsigned long long* counters, float* ray_bytes, hipStream_t st);
-// The primary pass as staged kernels (trace, grass refinement, shadow, voxel
-// shading, light gather, heightfield/sky) over pixel queues.
-hipError_t vx_launch_primary_staged(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb,
-                                    const vxd::PrimQueues& Q, hipStream_t st);
 hipError_t vx_launch_local_taa(const vxd::FrameBuffers& fb, hipStream_t st);
 hipError_t vx_launch_godray(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
                             unsigned long long* counters, hipStream_t st);

@@ -101,14 +101,10 @@ struct vx_view {
   float4* output = nullptr;
   float4* out_hist[2] = {nullptr, nullptr};
   vx_hit* hits = nullptr;
-  // Primary-pass stage buffers: voxel-hit records, pixel queues, counters.
   // One-view work-stealing launch: table [no hits, hits] and a small buffer
   // {0, 1, cursors[2 * 8]} whose first two words serve as the ping value.
   vxd::BatchView* ws_tab = nullptr;
   uint32_t* ws_buf = nullptr;
-  vxd::PrimRec* prim_rec = nullptr;
-  uint32_t* prim_q[vxd::kNumPrimQueues] = {};
-  uint32_t* prim_count = nullptr;
   unsigned long long* counters = nullptr;
   float* ray_bytes = nullptr;  // per ray: counted bytes of the last counting primary pass (allocated on first use)
   // Slot written last by each history-producing pass.
@@ -199,9 +195,6 @@ void free_view_buffers(vx_view* v) {
   dfree(v->pre_taa);
   dfree(v->output);
   dfree(v->hits);
-  dfree(v->prim_rec);
-  for (auto& q : v->prim_q) dfree(q);
-  dfree(v->prim_count);
   dfree(v->ws_tab);
   dfree(v->ws_buf);
   dfree(v->ray_bytes);
@@ -220,11 +213,6 @@ int alloc_view_buffers(vx_view* v) {
       return rc;
   }
   if ((rc = dalloc(&v->pre_taa, p)) || (rc = dalloc(&v->output, p)) || (rc = dalloc(&v->hits, r))) return rc;
-  if (v->ctx->flags & VX_CTX_PRIMARY_STAGED) {  // stage buffers only for the staged primary form (100 B/px)
-    if ((rc = dalloc(&v->prim_rec, r)) || (rc = dalloc(&v->prim_count, vxd::kNumPrimQueues))) return rc;
-    for (auto& q : v->prim_q)
-      if ((rc = dalloc(&q, r))) return rc;
-  }
   if (!v->cam_dev) return vx_fail(VX_E_INVALID, "view tables need the camera buffer");
   if ((rc = dalloc(&v->ws_tab, 2)) || (rc = dalloc(&v->ws_buf, 2 + 2 * vxd::kBatchBands))) return rc;
   const vxd::BatchView tab[2] = {one_view_entry(v, false), one_view_entry(v, true)};
@@ -264,6 +252,10 @@ int vx_create(const vx_caps* caps, vx_ctx** out) {
   vx_ctx* ctx = new (std::nothrow) vx_ctx();
   if (!ctx) return vx_fail(VX_E_NOMEM, "vx_create: host allocation");
   ctx->device = c.device;
+  if (c.flags & ~VX_CTX_GODRAY_MACRO_FIX) {
+    delete ctx;
+    return vx_fail(VX_E_INVALID, "vx_create: unknown flags 0x%x", c.flags);
+  }
   ctx->flags = c.flags;
   ctx->node_cap = c.node_capacity ? c.node_capacity : (1024ull * 1024 * 1024) / sizeof(vx_node);  // buffers.rs:79
   ctx->chunk_cap = c.chunk_capacity ? c.chunk_capacity : 57u * 4u * 57u;                           // buffers.rs:87-96
@@ -706,7 +698,7 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
     if (rc) return rc;
   }
   VX_HIP(hipEventRecord(v->ev[0], st));
-  if (!count && (mask & VX_PASS_PRIMARY) && !(c->flags & VX_CTX_PRIMARY_STAGED)) VX_HIP(vx_launch_ws_begin(B1, st));
+  if (!count && (mask & VX_PASS_PRIMARY)) VX_HIP(vx_launch_ws_begin(B1, st));
   if (mask & VX_PASS_PRIMARY) {
     if (count) {
       if (!v->ray_bytes) {
@@ -714,14 +706,8 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
         if (rc) return rc;
       }
       VX_HIP(vx_launch_primary(S, v->cam_dev, fb, true, v->counters, v->ray_bytes, st));
-    } else if (!(c->flags & VX_CTX_PRIMARY_STAGED)) {
-      VX_HIP(vx_launch_primary_ws(S, B1, persistent, st));
     } else {
-      vxd::PrimQueues Q;
-      Q.rec = v->prim_rec;
-      for (uint32_t i = 0; i < vxd::kNumPrimQueues; ++i) Q.q[i] = v->prim_q[i];
-      Q.count = v->prim_count;
-      VX_HIP(vx_launch_primary_staged(S, v->cam_dev, fb, Q, st));
+      VX_HIP(vx_launch_primary_ws(S, B1, persistent, st));
     }
     v->written_primary = pong;
   }

@@ -56,19 +56,6 @@ struct FrameBuffers {
   vx_hit* __restrict__ hits;
 };
 
-// The primary pass between its stages: one voxel-hit record per pixel and
-// five pixel queues (voxel hits, grass hits to refine, shadow pixels, pixels
-// that gather local light, pixels without a voxel hit) with their counters.
-struct PrimRec {
-  uint4 a, b, c, d, e;
-};
-enum PrimQueue : uint32_t { kQVox = 0, kQGrass = 1, kQShadow = 2, kQLight = 3, kQEnv = 4, kNumPrimQueues = 5 };
-struct PrimQueues {
-  PrimRec* __restrict__ rec;
-  uint32_t* __restrict__ q[kNumPrimQueues];
-  uint32_t* __restrict__ count;  // kNumPrimQueues counters
-};
-
 // A fused multi-view launch: one table entry per view (both history sets, so
 // the table stays valid while the views' ping-pong flips frame to frame),
 // prefix offsets of the views' work items, and the work-stealing cursors.

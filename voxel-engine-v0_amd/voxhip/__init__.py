@@ -47,7 +47,6 @@ OUT_RAY_BYTES = 10
 
 ARENA_NODES, ARENA_ROPES, ARENA_META, ARENA_MACRO, ARENA_COLINFO = range(5)
 CTX_GODRAY_MACRO_FIX = 1
-CTX_PRIMARY_STAGED = 2
 CAM_JITTER = 1
 CLIP_WRAP_FIX = 1
 SERVER_DEVICE_BUILD = 1
@@ -496,8 +495,8 @@ class Context:
     """Resident world on one GPU (Renderer::new + uploads)."""
 
     def __init__(self, node_capacity: int, chunk_capacity: int, grid_capacity: int, clip_levels: int = 5,
-                 clip_res: int = 256, device: int = 0, godray_macro_fix: bool = False, staged: bool = False):
-        flags = (CTX_GODRAY_MACRO_FIX if godray_macro_fix else 0) | (CTX_PRIMARY_STAGED if staged else 0)
+                 clip_res: int = 256, device: int = 0, godray_macro_fix: bool = False):
+        flags = CTX_GODRAY_MACRO_FIX if godray_macro_fix else 0
         caps = Caps(node_capacity, chunk_capacity, grid_capacity, clip_levels, clip_res, device, flags)
         h = C.c_void_p()
         _check(LIB.vx_create(C.byref(caps), C.byref(h)), "vx_create")
