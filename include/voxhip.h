@@ -276,6 +276,18 @@ int vx_write_history(vx_view* view, int which, const void* src, size_t bytes);
  * and texture bytes [read, written]. */
 int vx_view_counters(vx_view* view, uint64_t out[9], int reset);
 
+/* The primary pass's profile counters (ray_main.wgsl:32-74), raised while the
+ * camera's profile_flags & 1 by every primary pass of this view (vx_render
+ * and vx_batch_render): [PROFILE_VOXEL (candidate chunks per voxel trace),
+ * PROFILE_GRASS (grass refinement hits), PROFILE_HDR (shading calls, 2 per
+ * voxel hit and 1 per heightfield hit), PROFILE_FOG (fog weight
+ * max(1, 256 * clamp(t / 120, 0, 1))), PROFILE_SHADOW (2 per traced sun
+ * shadow, 1 per reused one)], u32 and wrapping like the reference's atomics.
+ * Waits for the device, copies them out and, when reset != 0, zeroes them.
+ * Replaces Renderer::read_primary_profile_counts_blocking and
+ * reset_primary_profile_counts (render/state/mod.rs:448-491). */
+int vx_view_profile(vx_view* view, uint32_t out[5], int reset);
+
 /* Per-pass GPU milliseconds of the last vx_render that ran them
  * (render/state/mod.rs:649-709): [primary, local_taa, godray, composite, composite_taa].
  * Waits for the view's stream. */

@@ -37,12 +37,19 @@ class Scene(C.Structure):
                 ("clip_height", C.c_void_p), ("n_clip", C.c_uint64)]
 
 
+PROFILE_NAMES = ("voxel", "grass", "hdr", "fog", "shadow")
+
+
 class Counts(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("grid", "meta", "macro", "node", "rope", "colinfo", "clip", "tex_rd",
-                                          "tex_wr", "oob")]
+                                          "tex_wr", "oob")] + [("prof", C.c_uint64 * 5)]
 
     def as_dict(self) -> dict:
-        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+        return {n: int(getattr(self, n)) for n, _ in self._fields_ if n != "prof"}
+
+    def profile(self) -> dict:
+        """The primary pass's profile counters (cam.profile_flags & 1), as u32 like the reference's."""
+        return {n: int(v) & 0xFFFFFFFF for n, v in zip(PROFILE_NAMES, self.prof)}
 
 
 def _load() -> C.CDLL:

@@ -84,6 +84,9 @@ constexpr float U32_TO_F01 = 1.0f / 4294967296.0f;
 struct Counts {
   uint64_t grid, meta, macro, node, rope, colinfo, clip;  // loads by array
   uint64_t tex_rd, tex_wr;                                 // history/texture bytes
+  // The primary pass's profile counters (ray_main.wgsl:32-47: PROFILE_VOXEL,
+  // GRASS, HDR, FOG, SHADOW), raised only when cam.profile_flags & 1.
+  uint64_t prof[5];
 };
 
 // ---- bindings -------------------------------------------------------------

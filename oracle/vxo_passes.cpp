@@ -471,13 +471,26 @@ static inline int32_t unpack_i16(uint32_t v) {
   int32_t s = (int32_t)(v & 0xFFFFu);
   return (v & 0x8000u) != 0u ? s - 0x10000 : s;
 }
+// profile_inc / profile_add, ray_main.wgsl:64-74 (counters of ray_main.wgsl:32-47)
+enum { PROFILE_VOXEL = 0, PROFILE_GRASS = 1, PROFILE_HDR = 2, PROFILE_FOG = 3, PROFILE_SHADOW = 4 };
+static inline void profile_add(const Ctx& C, uint32_t idx, uint32_t value) {
+  if ((C.cam->profile_flags & 1u) != 0u) C.cnt->prof[idx] += value;
+}
+// fog_profile_weight, ray_main.wgsl:90-94 (ENABLE_FOG is true)
+static inline uint32_t fog_profile_weight(float t_scene) {
+  float fog_norm = clampf(t_scene / FOG_MAX_DIST, 0.0f, 1.0f);
+  uint32_t w = f2u(fog_norm * 256.0f);
+  return w > 1u ? w : 1u;
+}
 struct HfShadeOut { v3 col; float t_scene; };
 // shade_heightfield, ray_main.wgsl:100-114
 static inline HfShadeOut shade_heightfield(const Ctx& C, v3 ro, v3 rd, const ClipHit& hf, v3 sky_up) {
+  profile_add(C, PROFILE_HDR, 1u);
   v3 surface = shade_clip_hit(C, ro, rd, hf, sky_up);
   float t_scene = mn(hf.t, FOG_MAX_DIST);
   v3 sbg = sky_bg(rd);
   v3 col = apply_fog(C, surface, ro, rd, t_scene, sbg);
+  profile_add(C, PROFILE_FOG, fog_profile_weight(t_scene));
   return HfShadeOut{col, t_scene};
 }
 // trace_primary_voxels, ray_main.wgsl:116-161
@@ -486,9 +499,11 @@ static inline VoxTraceResult trace_primary_voxels(const Ctx& C, const TileList& 
   VoxTraceResult vt{false, miss_hitgeom(), 0.0f, false, INVALID_U32, i3{0, 0, 0}};
   if (has_tile) {
     if (hist_valid) {
+      profile_add(C, PROFILE_VOXEL, tcc > 1u ? tcc : 1u);
       VoxTraceResult vh = trace_scene_voxels_candidates(C, L, ro, rd, t_start, t_end, hak != INVALID_U32, hac, hak, tcc);
       if (vh.best.hit != 0u) return vh;
     }
+    profile_add(C, PROFILE_VOXEL, tcc > 1u ? tcc : 1u);
     vt = trace_scene_voxels_candidates(C, L, ro, rd, 0.0f, FOG_MAX_DIST, false, i3{0, 0, 0}, INVALID_U32, tcc);
   }
   return vt;
@@ -569,6 +584,7 @@ static void main_primary_tile(const Ctx& C, const PrimaryIO& io, uint32_t wgx, u
             if (gh.hit) {
               hf.t = gh.t;
               hf.n = gh.n;
+              profile_add(C, PROFILE_GRASS, 1u);
             }
           }
           HfShadeOut sh = shade_heightfield(C, ro, rd, hf, sky_up);
@@ -650,12 +666,14 @@ static void main_primary_tile(const Ctx& C, const PrimaryIO& io, uint32_t wgx, u
         if (gh.hit) {
           best.t = gh.t;
           best.n = gh.n;
+          profile_add(C, PROFILE_GRASS, 1u);
         }
       }
       v3 hp = ro + best.t * rd;
       v3 hp_shadow = hp + best.n * (0.75f * vp_x(C));
       bool shadow_do = (seed & SHADOW_SUBSAMPLE_MASK) == 0u;
       if (shadow_do) {
+        profile_add(C, PROFILE_SHADOW, 2u);
         float shadow_hist = io.shadow_hist_in[pix];
         C.cnt->tex_rd += 4;
         v2 uv_prev = prev_uv_from_world(C, hp);
@@ -669,13 +687,16 @@ static void main_primary_tile(const Ctx& C, const PrimaryIO& io, uint32_t wgx, u
         float shadow_cur = sun_transmittance_geom_only(C, hp_shadow, SUN_DIR);
         shadow_out = mixf(shadow_hist, shadow_cur, SHADOW_TAA_ALPHA);
       } else {
+        profile_add(C, PROFILE_SHADOW, 1u);
         shadow_out = clampf(io.shadow_hist_in[pix], 0.0f, 1.0f);
         C.cnt->tex_rd += 4;
       }
       ShadeOut sh = shade_hit_split(C, ro, rd, best, sky_up, shadow_out);
+      profile_add(C, PROFILE_HDR, 2u);
       float t_scene = mn(best.t, FOG_MAX_DIST);
       v3 sbg = sky_bg(rd);
       v3 col_base = apply_fog(C, sh.base_hdr, ro, rd, t_scene, sbg);
+      profile_add(C, PROFILE_FOG, fog_profile_weight(t_scene));
       local_out = sh.local_hdr;
       local_w = sh.local_w;
       t_store = t_scene;
@@ -1053,6 +1074,7 @@ typedef struct vxo_scene {
 
 typedef struct vxo_counts {
   uint64_t grid, meta, macro, node, rope, colinfo, clip, tex_rd, tex_wr, oob;
+  uint64_t prof[5];  // PROFILE_VOXEL, GRASS, HDR, FOG, SHADOW (when cam.profile_flags & 1)
 } vxo_counts;
 
 }  // extern "C"
@@ -1117,6 +1139,7 @@ void add_counts(vxo_counts* dst, const vxo::Counts& c, uint64_t oob) {
   dst->tex_rd += c.tex_rd;
   dst->tex_wr += c.tex_wr;
   dst->oob += oob;
+  for (int i = 0; i < 5; ++i) dst->prof[i] += c.prof[i];
 }
 
 int hw_threads(int threads) {

@@ -26,3 +26,5 @@ inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
 inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
 inline int32_t min(int32_t a, int32_t b) { return a < b ? a : b; }
 inline int32_t max(int32_t a, int32_t b) { return a > b ? a : b; }
+// LDS atomics: each host task owns its workgroup's LDS, so a plain add is exact.
+inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { uint32_t o = *p; *p = o + v; return o; }

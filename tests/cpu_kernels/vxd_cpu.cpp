@@ -110,11 +110,13 @@ void fill_taper(float* taper) {
 
 extern "C" {
 
-// mode 0: the one-kernel pass (k_primary<false>); 2: the load-counting kernel
+// mode 0: the one-kernel pass (k_primary<false>), which adds the profile
+// counters into prof[5] when cam->profile_flags & 1 (as k_primary_batch does
+// through each tile's LDS counters); 2: the load-counting kernel
 // (k_primary<true>), which also adds its counts into counts[9].
 int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, uint32_t h, const float* primary_hist_in,
                     const float* shadow_in, float* color, float* depth, float* local, float* primary_hist_out,
-                    float* shadow_out, vx_hit* hits, int mode, int threads, uint64_t* counts) {
+                    float* shadow_out, vx_hit* hits, int mode, int threads, uint64_t* counts, uint64_t* prof) {
   if (!scene || !cam || !w || !h) return -1;
   const std::vector<uint32_t> lights = chunk_lights(scene);
   const Scene S = to_scene(scene, &lights);
@@ -132,6 +134,7 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
   const uint32_t tx = (w + 7u) / 8u, ty = (h + 7u) / 8u, nt = tx * ty;
   threads = std::max(1, threads);
   std::vector<Counts> cnt((size_t)threads, Counts{});
+  std::vector<uint64_t> prof_t((size_t)threads * 5, 0u);
 
   if (mode == 0 || mode == 2) {
     parallel_for(nt, threads, [&](uint32_t tile, int t) {
@@ -143,12 +146,15 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
         C.S = S;
         C.cam = cam;
         C.grass_taper = L.grass_taper;
+        for (uint32_t& v : L.prof) v = 0u;
+        C.prof = (cam->profile_flags & 1u) ? L.prof : nullptr;
         plain::tile_build(C, L, wgx, wgy);
         for (uint32_t lane = 0; lane < 64u; ++lane) {
           const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
           C.grass_blades = L.grass_blades + lane;
           if (gx < w && gy < h) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
         }
+        for (int i = 0; i < 5; ++i) prof_t[(size_t)t * 5 + i] += L.prof[i];
       } else {
         counted::Ctx C;
         C.S = S;
@@ -166,6 +172,9 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
   } else {
     return -1;
   }
+  if (prof)
+    for (int t = 0; t < threads; ++t)
+      for (int i = 0; i < 5; ++i) prof[i] += prof_t[(size_t)t * 5 + i];
   if (counts) {
     for (const Counts& c : cnt) {
       const uint32_t v[kNumCounts] = {c.grid, c.meta, c.macro, c.node, c.rope, c.colinfo, c.clip, c.tex_rd, c.tex_wr};

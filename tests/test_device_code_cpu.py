@@ -39,10 +39,15 @@ def test_device_code_matches_oracle(vx, name, res, frame_ids):
     ref = O.Frame(*res)
     frs = {m: O.Frame(*res) for m in vxd_cpu.MODES}
     for world, cam in _frames(name, res, frame_ids):
+        cam.profile_flags = 1  # the reference's profile counters (ray_main.wgsl:32-74) on both sides
         oc = O.Counts()
         O.primary(world, cam, ref, vx.HIT_DTYPE, counts=oc)
         for mode, fr in frs.items():
-            counts = vxd_cpu.primary(world, cam, fr, vx.HIT_DTYPE, mode=mode)
+            prof = np.zeros(5, np.uint64)
+            counts = vxd_cpu.primary(world, cam, fr, vx.HIT_DTYPE, mode=mode, profile=prof)
+            if mode == "one":
+                assert dict(zip(vx.PROFILE_NAMES, (int(v) for v in prof))) == oc.profile(), (prof, oc.profile())
+                assert all(v > 0 for v in oc.profile().values()), oc.profile()
             s = summary(fr.hits, ref.hits, fr.depth, ref.depth, fr.color, ref.color)
             assert s["identity_mismatches"] == 0 and s["t_bit_exact"] and s["color_bit_exact"], (mode, s)
             assert np.array_equal(fr.depth.view(np.uint32), ref.depth.view(np.uint32)), mode

@@ -287,3 +287,55 @@ def test_counted_loads_match_oracle(gpu):
     # 640x360 is a whole number of 8x8 tiles: the tile-gather shares add up exactly
     assert ray_bytes.sum() == gpu.algorithmic_bytes(c_prim)
     assert ray_bytes.min() > 0
+
+
+def test_profile_counters_match_oracle(gpu):
+    """The reference's five profile counters (ray_main.wgsl:32-74), raised while
+    the camera's profile_flags & 1: through vx_render and through a batch, equal
+    to the oracle's over three C1 flight frames at 640x360 (so windowed and
+    full traces, grass refinements, traced and reused shadows all occur); a
+    view in the same batch with the flag off stays at zero; reset clears them."""
+    O = _oracle()
+    from voxhip import scenes
+
+    sc, hw = host_world("c1")
+    clip, heights = scenes.scene_clipmap(sc, 0)
+    res = (640, 360)
+    ctx = make_context(gpu, hw)
+    ctx.write_clipmap(clip, heights)
+    single = gpu.View(ctx, res, res)
+    in_batch, quiet = gpu.View(ctx, res, res), gpu.View(ctx, (320, 180), (320, 180))
+    batch = gpu.Batch(ctx, [in_batch, quiet])
+    w = O.World.from_host(hw, clip, heights)
+    fr = O.Frame(*res)
+    prev = None
+    for f in range(3):
+        frame = 60 + f
+        cam = scenes.frame_camera(sc, hw, frame, prev_view_proj=prev, jitter=True, render=res)
+        prev = np.array(cam.view_proj, np.float32)
+        cam.profile_flags = 1
+        oc = O.Counts()
+        O.primary(w, cam, fr, counts=oc)   # the oracle's pipeline, with the primary pass counted
+        O.local_taa(fr)
+        O.godray(w, cam, fr)
+        O.composite(cam, fr)
+        O.composite_taa(cam, fr)
+        ref = oc.profile()
+        single.write_camera(cam)
+        single.render(gpu.PASS_FRAME)
+        in_batch.write_camera(cam)
+        quiet.write_camera(scenes.frame_camera(sc, hw, frame, render=(320, 180)))   # profile_flags 0
+        batch.render(gpu.PASS_FRAME)
+        got_single, got_batch, got_quiet = single.profile(), in_batch.profile(), quiet.profile()
+        print(f"[parity] profile counters frame {frame}: oracle {ref}, vx_render {got_single}, batch {got_batch}",
+              flush=True)
+        assert got_single == ref and got_batch == ref, (ref, got_single, got_batch)
+        assert all(v == 0 for v in got_quiet.values()), got_quiet
+        assert all(v > 0 for v in ref.values()), ref
+        assert np.array_equal(single.read(gpu.OUT_OUTPUT).view(np.uint32), fr.output.view(np.uint32))
+    assert all(v == 0 for v in single.profile().values())   # the reads above reset them
+    single.render(gpu.PASS_PRIMARY)   # the same camera again, no reset in between: the counters add up
+    single.render(gpu.PASS_PRIMARY)
+    twice = single.profile(reset=False)
+    assert twice == single.profile() and all(v > 0 for v in twice.values())
+    batch.close()

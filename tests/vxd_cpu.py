@@ -43,7 +43,7 @@ def lib():
         _LIB = C.CDLL(str(build()))
         vp = C.c_void_p
         _LIB.vxd_cpu_primary.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,
-                                         vp]
+                                         vp, vp]
         _LIB.vxd_cpu_primary.restype = C.c_int
         _LIB.vxd_cpu_query.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_int]
         _LIB.vxd_cpu_query.restype = C.c_int
@@ -56,8 +56,9 @@ def lib():
     return _LIB
 
 
-def primary(world, cam, fr, hits_dtype, mode: str = "one", threads: int = 0):
-    """Same contract as oracle.primary: updates fr in place; returns the load counts for mode 'counted'."""
+def primary(world, cam, fr, hits_dtype, mode: str = "one", threads: int = 0, profile: np.ndarray | None = None):
+    """Same contract as oracle.primary: updates fr in place; returns the load counts for mode 'counted'.
+    profile (5 uint64, mode 'one'): receives the profile counters when cam.profile_flags & 1."""
     threads = threads or min(16, os.cpu_count() or 1)
     hist_out = np.zeros_like(fr.primary_hist)
     shadow_out = np.zeros_like(fr.shadow)
@@ -66,7 +67,7 @@ def primary(world, cam, fr, hits_dtype, mode: str = "one", threads: int = 0):
     rc = lib().vxd_cpu_primary(C.addressof(world.scene), C.addressof(cam), fr.rw, fr.rh, fr.primary_hist.ctypes.data,
                                fr.shadow.ctypes.data, fr.color.ctypes.data, fr.depth.ctypes.data, fr.local.ctypes.data,
                                hist_out.ctypes.data, shadow_out.ctypes.data, hits.ctypes.data, MODES[mode], threads,
-                               counts.ctypes.data)
+                               counts.ctypes.data, None if profile is None else profile.ctypes.data)
     assert rc == 0
     fr.primary_hist, fr.shadow, fr.hits = hist_out, shadow_out, hits
     return counts

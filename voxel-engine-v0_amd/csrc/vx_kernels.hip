@@ -214,9 +214,13 @@ __global__ __launch_bounds__(64) void k_batch_end(Batch B) {
   for (uint32_t v = threadIdx.x; v < B.n; v += 64u) B.ping[v] ^= 1u;
 }
 
+// PROF: the launch carries a view whose camera has profile_flags & 1; the
+// product launch (PROF = false) has no profiling code at all.
+template <bool PROF>
 __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scene S, Batch B) {
   __shared__ TileLDS L;
   if (threadIdx.x < 12u) L.grass_taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
+  if (threadIdx.x < 5u) L.prof[threadIdx.x] = 0u;
   uint32_t band = blockIdx.x % kBatchBands, steps = 0u, t0 = 0u, t1 = 0u;
   while (batch_take(B, 0u, band, steps, t0, t1)) {
     for (uint32_t t = t0; t < t1; ++t) {
@@ -230,11 +234,23 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
       C.cam = V.cam;
       C.grass_taper = L.grass_taper;
       C.grass_blades = L.grass_blades + threadIdx.x;
+      // Profiling (ray_main.wgsl:64-74) is a property of the view's camera, so
+      // uniform over the tile: the counters go to LDS and are flushed per tile.
+      const bool prof = PROF && (V.cam->profile_flags & 1u) != 0u;
+      C.prof = prof ? L.prof : nullptr;
       __syncthreads();  // the previous tile's lanes are done with L
       if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
       __syncthreads();
       const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
       if (gx < fb.rw && gy < fb.rh) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+      if (prof) {
+        __syncthreads();  // every lane's LDS increments are in
+        if (threadIdx.x < 5u) {
+          const uint32_t v = L.prof[threadIdx.x];
+          if (v) atomicAdd(fb.prof + threadIdx.x, v);
+          L.prof[threadIdx.x] = 0u;
+        }
+      }
     }
   }
 }
@@ -389,9 +405,12 @@ hipError_t vx_launch_composite_taa(const vxd::Scene& S, const vx_camera* cam_dev
 // Fused passes of a batch in reference order; `persistent` is the number of
 // workgroups of the two work-stealing kernels.
 hipError_t vx_launch_batch(const vxd::Scene& S, const vxd::Scene& S_godray, const vxd::Batch& B, uint32_t mask,
-                           uint32_t persistent, hipStream_t st) {
+                           uint32_t persistent, bool profile, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, B);
-  if (mask & VX_PASS_PRIMARY) hipLaunchKernelGGL(k_primary_batch, dim3(persistent), dim3(64), 0, st, S, B);
+  if (mask & VX_PASS_PRIMARY) {
+    if (profile) hipLaunchKernelGGL(k_primary_batch<true>, dim3(persistent), dim3(64), 0, st, S, B);
+    else hipLaunchKernelGGL(k_primary_batch<false>, dim3(persistent), dim3(64), 0, st, S, B);
+  }
   if (mask & VX_PASS_LOCAL_TAA) hipLaunchKernelGGL(k_local_taa_batch, dim3(B.rblks), dim3(256), 0, st, B);
   if (mask & VX_PASS_GODRAY) hipLaunchKernelGGL(k_godray_batch, dim3(B.tiles), dim3(64), 0, st, S_godray, B);
   if (mask & VX_PASS_COMPOSITE) hipLaunchKernelGGL(k_composite_batch, dim3(B.pblks), dim3(256), 0, st, S, B);
@@ -405,8 +424,10 @@ hipError_t vx_launch_ws_begin(const vxd::Batch& B, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, B);
   return hipGetLastError();
 }
-hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, hipStream_t st) {
-  hipLaunchKernelGGL(k_primary_batch, dim3(persistent), dim3(64), 0, st, S, B);
+hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, bool profile,
+                                hipStream_t st) {
+  if (profile) hipLaunchKernelGGL(k_primary_batch<true>, dim3(persistent), dim3(64), 0, st, S, B);
+  else hipLaunchKernelGGL(k_primary_batch<false>, dim3(persistent), dim3(64), 0, st, S, B);
   return hipGetLastError();
 }
 

@@ -16,12 +16,15 @@ hipError_t vx_launch_composite(const vxd::Scene& S, const vx_camera* cam_dev, co
 hipError_t vx_launch_composite_taa(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, hipStream_t st);
 // A batch of views: fused work-stealing primary and godray kernels, flattened
 // streaming passes, and the end-of-frame ping flip, in reference pass order.
+// profile: launch the primary kernel that raises the profile counters of the
+// views whose camera has profile_flags & 1 (the other form has no such code).
 hipError_t vx_launch_batch(const vxd::Scene& S, const vxd::Scene& S_godray, const vxd::Batch& B, uint32_t mask,
-                           uint32_t persistent, hipStream_t st);
+                           uint32_t persistent, bool profile, hipStream_t st);
 // One view through the same work-stealing primary kernel: reset the tile
 // cursors, then a persistent launch of `persistent` workgroups.
 hipError_t vx_launch_ws_begin(const vxd::Batch& B, hipStream_t st);
-hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, hipStream_t st);
+hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, bool profile,
+                                hipStream_t st);
 // Ray queries over n rays (device arrays) as one persistent launch with lane refill.
 hipError_t vx_launch_query(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,
                            uint32_t* cursor, uint32_t persistent, uint32_t refill_below, hipStream_t st);

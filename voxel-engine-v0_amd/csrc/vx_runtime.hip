@@ -106,6 +106,8 @@ struct vx_view {
   vxd::BatchView* ws_tab = nullptr;
   uint32_t* ws_buf = nullptr;
   unsigned long long* counters = nullptr;
+  uint32_t* prof = nullptr;  // the five profile counters (ray_main.wgsl:32-47), kept across resizes
+  uint32_t profile_flags = 0;  // of the camera last written (the launch picks the profiling kernel by it)
   float* ray_bytes = nullptr;  // per ray: counted bytes of the last counting primary pass (allocated on first use)
   // Slot written last by each history-producing pass.
   int written_primary = 1, written_local = 1, written_godray = 1, written_out = 1;
@@ -455,8 +457,10 @@ int vx_view_create(vx_ctx* ctx, uint32_t pw, uint32_t ph, uint32_t rw, uint32_t 
   v->rw = rw;
   v->rh = rh;
   DeviceGuard g(ctx->device);
-  // The camera buffer first: the view's kernel table (built with the buffers) points at it.
+  // The camera buffer and the profile counters first: the view's kernel table
+  // (built with the buffers) points at them.
   int rc = dalloc(&v->cam_dev, 1);
+  if (!rc) rc = dalloc(&v->prof, 5);
   if (!rc) rc = alloc_view_buffers(v);
   if (!rc) rc = dalloc(&v->counters, 16);
   for (int i = 0; i < 10 && !rc; ++i)
@@ -475,6 +479,7 @@ void vx_view_destroy(vx_view* v) {
   hipStreamSynchronize(v->ctx->stream);
   free_view_buffers(v);
   dfree(v->cam_dev);
+  dfree(v->prof);
   dfree(v->counters);
   for (int i = 0; i < 10; ++i)
     if (v->ev[i]) hipEventDestroy(v->ev[i]);
@@ -519,6 +524,7 @@ int vx_write_camera(vx_view* v, const vx_camera* cam) {
   DeviceGuard g(v->ctx->device);
   VX_HIP(hipMemcpyAsync(v->cam_dev, cam, sizeof(vx_camera), hipMemcpyHostToDevice, v->ctx->stream));
   VX_HIP(hipStreamSynchronize(v->ctx->stream));
+  v->profile_flags = cam->profile_flags;
   return VX_OK;
 }
 
@@ -589,6 +595,7 @@ vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits) {
   fb.out_hist_out = v->out_hist[pong];
   fb.output = v->output;
   fb.hits = hits ? v->hits : nullptr;
+  fb.prof = v->prof;
   return fb;
 }
 
@@ -707,7 +714,7 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
       }
       VX_HIP(vx_launch_primary(S, v->cam_dev, fb, true, v->counters, v->ray_bytes, st));
     } else {
-      VX_HIP(vx_launch_primary_ws(S, B1, persistent, st));
+      VX_HIP(vx_launch_primary_ws(S, B1, persistent, (v->profile_flags & 1u) != 0u, st));
     }
     v->written_primary = pong;
   }
@@ -809,6 +816,18 @@ int vx_view_counters(vx_view* v, uint64_t out[9], int reset) {
   return VX_OK;
 }
 
+int vx_view_profile(vx_view* v, uint32_t out[5], int reset) {
+  if (!v || !out) return vx_fail(VX_E_INVALID, "vx_view_profile: null argument");
+  DeviceGuard g(v->ctx->device);
+  VX_HIP(hipDeviceSynchronize());  // single-view and batch renders, on any stream
+  VX_HIP(hipMemcpy(out, v->prof, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (reset) {
+    const uint32_t zero[5] = {0u, 0u, 0u, 0u, 0u};
+    VX_HIP(hipMemcpy(v->prof, zero, sizeof(zero), hipMemcpyHostToDevice));
+  }
+  return VX_OK;
+}
+
 int vx_view_timings(vx_view* v, float ms[5]) {
   if (!v || !ms) return vx_fail(VX_E_INVALID, "vx_view_timings: null argument");
   DeviceGuard g(v->ctx->device);
@@ -903,9 +922,12 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
     const int rc = refresh_chunk_lights(c, st);
     if (rc) return rc;
   }
+  bool profile = false;
+  for (const vx_view* v : b->views) profile |= (v->profile_flags & 1u) != 0u;
   VX_HIP(hipEventRecord(b->ev[0], st));
   if (b->cfg.use_graph) {
-    const uint64_t key = ((uint64_t)mask << 32) | ((uint64_t)b->cfg.waves_per_simd << 16) | b->cfg.grab;
+    const uint64_t key = ((uint64_t)profile << 62) | ((uint64_t)mask << 32) | ((uint64_t)b->cfg.waves_per_simd << 16) |
+                         b->cfg.grab;
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
       hipGraph_t graph = nullptr;
@@ -914,7 +936,7 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
       VX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
       if (e == hipSuccess) {
-        hipError_t le = vx_launch_batch(S, SG, b->B, mask, persistent, cap);
+        hipError_t le = vx_launch_batch(S, SG, b->B, mask, persistent, profile, cap);
         e = hipStreamEndCapture(cap, &graph);
         if (e == hipSuccess) e = le;
       }
@@ -926,7 +948,7 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
     }
     VX_HIP(hipGraphLaunch(it->second, st));
   } else {
-    VX_HIP(vx_launch_batch(S, SG, b->B, mask, persistent, st));
+    VX_HIP(vx_launch_batch(S, SG, b->B, mask, persistent, profile, st));
   }
   VX_HIP(hipEventRecord(b->ev[1], st));
   for (uint32_t i = 0; i < n; ++i) {

@@ -54,6 +54,9 @@ struct FrameBuffers {
   float4* __restrict__ out_hist_out;
   float4* __restrict__ output;
   vx_hit* __restrict__ hits;
+  // The view's five profile counters (ray_main.wgsl:32-47), raised by the
+  // primary pass when the camera's profile_flags & 1.
+  uint32_t* __restrict__ prof;
 };
 
 // A fused multi-view launch: one table entry per view (both history sets, so
@@ -89,6 +92,7 @@ struct TileLDS {
   float enter[kMaxTileChunks];
   float grass_taper[12];  // blade taper per (lod, segment), see grass_taper_entry
   float grass_blades[kGrassBladeCache];  // per-lane blade cache, see grass_cache_store
+  uint32_t prof[5];  // the tile's profile counter increments, flushed to FrameBuffers::prof per tile
 };
 
 }  // namespace vxd

@@ -62,6 +62,7 @@ META_DTYPE = np.dtype(
 HIT_DTYPE = np.dtype(
     [("slot", "<u4"), ("node", "<u4"), ("voxel", "<i4", (3,)), ("mat", "<u4"), ("t", "<f4"), ("kind", "<u4")]
 )
+PROFILE_NAMES = ("voxel", "grass", "hdr", "fog", "shadow")
 COUNT_NAMES = ("grid", "meta", "macro", "node", "rope", "colinfo", "clip", "tex_rd", "tex_wr")
 # Bytes per counted element (gpu_types.rs struct sizes; textures count bytes).
 COUNT_BYTES = (4, 48, 4, 16, 32, 4, 4, 1, 1)
@@ -204,6 +205,7 @@ def _load() -> C.CDLL:
         "vx_write_history": ([vp, C.c_int, vp, C.c_size_t], C.c_int),
         "vx_view_counters": ([vp, C.POINTER(u64), C.c_int], C.c_int),
         "vx_view_timings": ([vp, C.POINTER(f32)], C.c_int),
+        "vx_view_profile": ([vp, C.POINTER(u32), C.c_int], C.c_int),
         "vx_query_rays": ([vp, C.POINTER(Camera), vp, u32, vp, vp], C.c_int),
         "vx_query_timing": ([vp, C.POINTER(f32)], C.c_int),
         "vx_query_set_config": ([vp, C.POINTER(QueryConfig)], C.c_int),
@@ -256,7 +258,7 @@ LIB = _load()
 EXPORTED = (
     "vx_last_error", "vx_version", "vx_device_count", "vx_create", "vx_destroy", "vx_upload_regions", "vx_write_grid",
     "vx_write_clipmap", "vx_view_create", "vx_view_destroy", "vx_view_resize", "vx_view_reset", "vx_write_camera",
-    "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings",
+    "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings", "vx_view_profile",
     "vx_server_create", "vx_server_destroy", "vx_server_submit", "vx_server_poll", "vx_server_edit",
     "vx_server_wait_edits", "vx_server_drain", "vx_server_close_stream", "vx_server_get_stats",
     "vx_query_rays", "vx_query_timing", "vx_query_set_config", "vx_query_get_config", "vx_query_tune", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
@@ -617,6 +619,12 @@ class View:
         out = (C.c_uint64 * 9)()
         _check(LIB.vx_view_counters(self._h, out, 1 if reset else 0), "vx_view_counters")
         return dict(zip(COUNT_NAMES, (int(v) for v in out)))
+
+    def profile(self, reset: bool = True) -> dict:
+        """The primary pass's profile counters (vx_view_profile)."""
+        out = (C.c_uint32 * 5)()
+        _check(LIB.vx_view_profile(self._h, out, 1 if reset else 0), "vx_view_profile")
+        return dict(zip(PROFILE_NAMES, (int(v) for v in out)))
 
     def timings(self) -> list[float]:
         ms = (C.c_float * 5)()
