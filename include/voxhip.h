@@ -355,6 +355,22 @@ int  vx_batch_get_config(const vx_batch* batch, vx_batch_config* cfg);
  * keeps the fastest configuration and returns it in *chosen (may be NULL). */
 int  vx_batch_tune(vx_batch* batch, uint32_t pass_mask, uint32_t iters, vx_batch_config* chosen);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Renders up to VX_NOALLOC_MAX_VIEWS views of ctx as one fused
+ * launch sequence, exactly as a vx_batch over the same views would
+ * (vx_batch_render, pass_mask likewise without hits or counting), but with no
+ * vx_batch object and without allocating: the launch's view table, ping words
+ * and tile cursors live in a static device array (about 24 KB) that each GPU
+ * keeps for the life of the process, resolved once per device by vx_create.
+ * More views than VX_NOALLOC_MAX_VIEWS fail with VX_E_CAPACITY.  Every call on
+ * one GPU uses the same array (across contexts): calls that overlap on a GPU,
+ * from several threads or on several streams, race and must be serialised by
+ * the caller.  The call waits for its table upload on `stream` (a hipStream_t
+ * or NULL for the context's stream) and queues the passes without waiting. */
+#define VX_NOALLOC_MAX_VIEWS 64u
+int vx_render_views_noalloc(vx_ctx* ctx, vx_view* const* views, uint32_t n, uint32_t pass_mask, void* stream);
+
 /* Blocks until all work queued on the context's stream is done. */
 int vx_sync(vx_ctx* ctx);
 

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <vector>
@@ -39,6 +40,36 @@ int vx_fail(int code, const char* fmt, ...) {
     hipError_t e_ = (call);                                                                   \
     if (e_ != hipSuccess) return vx_fail(VX_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
+
+// Scratch of vx_render_views_noalloc, one copy per GPU for the life of the
+// process: the fused launch's view table, the views' ping words and the
+// per-band tile cursors, sized for VX_NOALLOC_MAX_VIEWS views.  Host code
+// never uses the symbol itself as an address: vx_create resolves it once per
+// device with hipGetSymbolAddress (with that device current) and the entry
+// point passes the resolved pointer to the copies and kernels.
+namespace {
+struct NoAllocScratch {
+  vxd::BatchView table[VX_NOALLOC_MAX_VIEWS];
+  uint32_t ping[VX_NOALLOC_MAX_VIEWS];
+  uint32_t cursor[2 * vxd::kBatchBands];
+};
+static __device__ __align__(256) unsigned char scratch[sizeof(NoAllocScratch)];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+NoAllocScratch* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves this device's copy of `scratch` once (the caller has made `dev` current).
+int resolve_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return vx_fail(VX_E_INVALID, "device %d out of range for the static scratch", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return VX_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return vx_fail(VX_E_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  g_scratch_ptr[dev] = static_cast<NoAllocScratch*>(p);
+  return VX_OK;
+}
+}  // namespace
 
 struct vx_ctx {
   int device = 0;
@@ -286,6 +317,10 @@ int vx_create(const vx_caps* caps, vx_ctx** out) {
       hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     vx_destroy(ctx);
     return vx_fail(VX_E_DEVICE, "vx_create: stream/copy failed");
+  }
+  if (const int rc = resolve_scratch(ctx->device)) {
+    vx_destroy(ctx);
+    return rc;
   }
   *out = ctx;
   return VX_OK;
@@ -603,18 +638,14 @@ vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits) {
 // are taken first, small views fill the tail) and the ping mirror, uploaded
 // before this returns.  The device arrays grow only when n exceeds their
 // capacity; graphs captured for the old table are dropped.
-int batch_bind(vx_batch* b, vx_view* const* views, uint32_t n) {
-  drop_graphs(b);
-  b->views.assign(views, views + n);
-  std::stable_sort(b->views.begin(), b->views.end(),
-                   [](const vx_view* a, const vx_view* c) { return (uint64_t)a->rw * a->rh > (uint64_t)c->rw * c->rh; });
-  b->tab_host.resize(n);
-  b->gens.clear();
-  b->ping_host.clear();
-  uint32_t tiles = 0, rblks = 0, pblks = 0;
+// Table entries of a fused launch over `views` (in table order) and the
+// launch totals: tiles, render blocks, present blocks.
+void batch_entries(vx_view* const* views, uint32_t n, vxd::BatchView* tab, uint32_t& tiles, uint32_t& rblks,
+                   uint32_t& pblks) {
+  tiles = rblks = pblks = 0;
   for (uint32_t i = 0; i < n; ++i) {
-    const vx_view* v = b->views[i];
-    vxd::BatchView& e = b->tab_host[i];
+    const vx_view* v = views[i];
+    vxd::BatchView& e = tab[i];
     std::memset(&e, 0, sizeof(e));
     e.fb[0] = view_buffers(v, 0, false);
     e.fb[1] = view_buffers(v, 1, false);
@@ -631,6 +662,21 @@ int batch_bind(vx_batch* b, vx_view* const* views, uint32_t n) {
     tiles += e.tiles;
     rblks += e.rblks;
     pblks += e.pblks;
+  }
+}
+
+bool larger_render(const vx_view* a, const vx_view* c) { return (uint64_t)a->rw * a->rh > (uint64_t)c->rw * c->rh; }
+
+int batch_bind(vx_batch* b, vx_view* const* views, uint32_t n) {
+  drop_graphs(b);
+  b->views.assign(views, views + n);
+  std::stable_sort(b->views.begin(), b->views.end(), larger_render);
+  b->tab_host.resize(n);
+  b->gens.clear();
+  b->ping_host.clear();
+  uint32_t tiles = 0, rblks = 0, pblks = 0;
+  batch_entries(b->views.data(), n, b->tab_host.data(), tiles, rblks, pblks);
+  for (const vx_view* v : b->views) {
     b->gens.push_back(v->gen);
     b->ping_host.push_back((uint32_t)v->ping);
   }
@@ -972,6 +1018,61 @@ int vx_batch_timing(vx_batch* b, float* ms) {
   DeviceGuard g(b->ctx->device);
   VX_HIP(hipEventSynchronize(b->ev[1]));
   VX_HIP(hipEventElapsedTime(ms, b->ev[0], b->ev[1]));
+  return VX_OK;
+}
+
+int vx_render_views_noalloc(vx_ctx* c, vx_view* const* views, uint32_t n, uint32_t mask, void* stream) {
+  if (!c || (!views && n)) return vx_fail(VX_E_INVALID, "vx_render_views_noalloc: null argument");
+  if (n > VX_NOALLOC_MAX_VIEWS)
+    return vx_fail(VX_E_CAPACITY, "vx_render_views_noalloc: %u views, at most %u (the static scratch's size)", n,
+                   (unsigned)VX_NOALLOC_MAX_VIEWS);
+  if (mask & (VX_PASS_WRITE_HITS | VX_PASS_COUNT))
+    return vx_fail(VX_E_INVALID, "vx_render_views_noalloc: hits and counting run per view");
+  if (!n) return VX_OK;
+  if (int rc = check_batch_views(c, views, n, "vx_render_views_noalloc")) return rc;
+  DeviceGuard g(c->device);
+  NoAllocScratch* sc = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+  if (!sc) return vx_fail(VX_E_DEVICE, "vx_render_views_noalloc: the static scratch of device %d is not resolved", c->device);
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  // Table order: largest render first (stable), in fixed-size host arrays.
+  vx_view* order[VX_NOALLOC_MAX_VIEWS];
+  for (uint32_t i = 0; i < n; ++i) order[i] = views[i];
+  for (uint32_t i = 1; i < n; ++i)
+    for (uint32_t j = i; j > 0 && larger_render(order[j], order[j - 1]); --j) std::swap(order[j], order[j - 1]);
+  vxd::BatchView tab[VX_NOALLOC_MAX_VIEWS];
+  uint32_t ping[VX_NOALLOC_MAX_VIEWS];
+  vxd::Batch B{};
+  batch_entries(order, n, tab, B.tiles, B.rblks, B.pblks);
+  bool profile = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    ping[i] = (uint32_t)order[i]->ping;
+    profile |= (order[i]->profile_flags & 1u) != 0u;
+  }
+  // The copies read host memory: wait for them before the host arrays go away.
+  VX_HIP(hipMemcpyAsync(sc->table, tab, n * sizeof(vxd::BatchView), hipMemcpyHostToDevice, st));
+  VX_HIP(hipMemcpyAsync(sc->ping, ping, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  VX_HIP(hipStreamSynchronize(st));
+  B.views = sc->table;
+  B.ping = sc->ping;
+  B.cursor = sc->cursor;   // cleared by the launch's first kernel
+  B.n = n;
+  B.grab = 1u;
+  if (mask & VX_PASS_PRIMARY) {
+    const int rc = refresh_chunk_lights(c, st);
+    if (rc) return rc;
+  }
+  const uint32_t persistent = std::min(persistent_groups(vx_batch_config{6u, 1u, 0u, 0u}), std::max(B.tiles, 1u));
+  VX_HIP(vx_launch_batch(make_scene(c, false), make_scene(c, true), B, mask, persistent, profile, st));
+  for (uint32_t i = 0; i < n; ++i) {
+    vx_view* v = order[i];
+    const int pong = 1 - v->ping;
+    if (mask & VX_PASS_PRIMARY) v->written_primary = pong;
+    if (mask & VX_PASS_LOCAL_TAA) v->written_local = pong;
+    if (mask & VX_PASS_GODRAY) v->written_godray = pong;
+    if (mask & VX_PASS_COMPOSITE_TAA) v->written_out = pong;
+    v->last_mask = mask;
+    if (mask & VX_PASS_END_FRAME) v->ping = pong;
+  }
   return VX_OK;
 }
 

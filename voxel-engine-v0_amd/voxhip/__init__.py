@@ -24,6 +24,7 @@ import numpy as np
 LIB_PATH = Path(os.environ.get("VOXHIP_LIB") or (Path(__file__).resolve().parent / "libvoxhip.so"))
 
 VX_OK = 0
+VX_E_CAPACITY = -2
 VX_E_BUSY = -6
 VX_E_AGAIN = -7
 REQ_VIEW, REQ_RAYS = 1, 2
@@ -40,6 +41,7 @@ PASS_END_FRAME = 0x80
 PASS_FRAME = 0x9F
 PASS_WRITE_HITS = 0x100
 PASS_COUNT = 0x200
+NOALLOC_MAX_VIEWS = 64
 
 OUT_COLOR, OUT_DEPTH, OUT_LOCAL, OUT_PRIMARY_HIST, OUT_SHADOW_HIST = 0, 1, 2, 3, 4
 OUT_LOCAL_HIST, OUT_GODRAY, OUT_PRE_TAA, OUT_OUTPUT, OUT_HITS = 5, 6, 7, 8, 9
@@ -206,6 +208,7 @@ def _load() -> C.CDLL:
         "vx_view_counters": ([vp, C.POINTER(u64), C.c_int], C.c_int),
         "vx_view_timings": ([vp, C.POINTER(f32)], C.c_int),
         "vx_view_profile": ([vp, C.POINTER(u32), C.c_int], C.c_int),
+        "vx_render_views_noalloc": ([vp, C.POINTER(vp), u32, u32, vp], C.c_int),
         "vx_query_rays": ([vp, C.POINTER(Camera), vp, u32, vp, vp], C.c_int),
         "vx_query_timing": ([vp, C.POINTER(f32)], C.c_int),
         "vx_query_set_config": ([vp, C.POINTER(QueryConfig)], C.c_int),
@@ -258,7 +261,7 @@ LIB = _load()
 EXPORTED = (
     "vx_last_error", "vx_version", "vx_device_count", "vx_create", "vx_destroy", "vx_upload_regions", "vx_write_grid",
     "vx_write_clipmap", "vx_view_create", "vx_view_destroy", "vx_view_resize", "vx_view_reset", "vx_write_camera",
-    "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings", "vx_view_profile",
+    "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings", "vx_view_profile", "vx_render_views_noalloc",
     "vx_server_create", "vx_server_destroy", "vx_server_submit", "vx_server_poll", "vx_server_edit",
     "vx_server_wait_edits", "vx_server_drain", "vx_server_close_stream", "vx_server_get_stats",
     "vx_query_rays", "vx_query_timing", "vx_query_set_config", "vx_query_get_config", "vx_query_tune", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
