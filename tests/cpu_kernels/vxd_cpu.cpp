@@ -148,7 +148,17 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
         C.grass_taper = L.grass_taper;
         for (uint32_t& v : L.prof) v = 0u;
         C.prof = (cam->profile_flags & 1u) ? L.prof : nullptr;
-        plain::tile_build(C, L, wgx, wgy);
+        // k_primary_batch's wavefront gather: each phase for every lane, the
+        // barrier between phases being the end of each lane loop.
+        for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_walk(C, L, wgx, wgy, lane);
+        if (plain::gather_overflow(L)) {
+          plain::tile_build(C, L, wgx, wgy);
+        } else {
+          for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_load(C, L, lane);
+          for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_first(L, lane);
+          for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_place(L, lane);
+          plain::gather_finish(L);
+        }
         for (uint32_t lane = 0; lane < 64u; ++lane) {
           const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
           C.grass_blades = L.grass_blades + lane;
@@ -253,6 +263,49 @@ int vxd_cpu_math(const float* x, const float* y, uint32_t n, float* out) {
     out[5 * k + 3] = vx_log(x[k]);
     out[5 * k + 4] = vx_pow(x[k], y[k]);
   }
+  return 0;
+}
+
+// The tile candidate lists of a w x h view: mode 0 the one-lane gather
+// (tile_build), mode 1 the wavefront phases of k_primary_batch.  Per tile:
+// count (the raw counter), cached, and the first 24 slots and enter t.
+int vxd_cpu_tile_lists(const cpu_scene* scene, const vx_camera* cam, uint32_t w, uint32_t h, int mode, uint32_t* counts,
+                       uint32_t* cached, uint32_t* slots, float* enter, uint32_t* overflow) {
+  if (!scene || !cam || !w || !h) return -1;
+  const std::vector<uint32_t> lights = chunk_lights(scene);
+  const Scene S = to_scene(scene, &lights);
+  const uint32_t tx = (w + 7u) / 8u, ty = (h + 7u) / 8u;
+  uint32_t over = 0;
+  for (uint32_t tile = 0; tile < tx * ty; ++tile) {
+    TileLDS L;
+    std::memset(&L, 0xAB, sizeof(L));  // stale contents, as LDS holds between tiles
+    plain::Ctx C;
+    C.S = S;
+    C.cam = cam;
+    C.grass_taper = L.grass_taper;
+    const uint32_t wgx = tile % tx, wgy = tile / tx;
+    if (mode == 0) {
+      plain::tile_build(C, L, wgx, wgy);
+    } else {
+      for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_walk(C, L, wgx, wgy, lane);
+      if (plain::gather_overflow(L)) {
+        ++over;
+        plain::tile_build(C, L, wgx, wgy);
+      } else {
+        for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_load(C, L, lane);
+        for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_first(L, lane);
+        for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_place(L, lane);
+        plain::gather_finish(L);
+      }
+    }
+    counts[tile] = L.count;
+    cached[tile] = L.cached;
+    for (uint32_t i = 0; i < 24u; ++i) {
+      slots[tile * 24u + i] = i < L.cached ? L.slots[i] : 0u;
+      enter[tile * 24u + i] = i < L.cached ? L.enter[i] : 0.0f;
+    }
+  }
+  if (overflow) *overflow = over;
   return 0;
 }
 

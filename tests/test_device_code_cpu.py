@@ -187,3 +187,46 @@ def test_constant_divisor_check_catches_a_wrong_method():
         plain = (x * np.float32(np.float32(1.0) / np.float32(d))).astype(np.float32)
         exact = (x / np.float32(d)).astype(np.float32)
         assert np.any(plain != exact), d
+
+
+def test_wavefront_gather_equals_one_lane_gather(vx):
+    """The product kernel's tile gather over the whole wavefront (walk, parallel
+    loads, first occurrences, prefix placement, then the reference's selection
+    sort) gives the one-lane gather's list exactly: raw count, cached count,
+    slots and enter t of the first 24, on cameras that reach the ties and the
+    truncation: on a chunk corner looking along axes and diagonals (both
+    probe rays start in different chunks at the same enter t), low over the
+    ground looking diagonally (more than 24 candidates), inside and outside
+    the grid, and at random poses."""
+    import math
+
+    from oracle import oracle as O
+    from voxhip import scenes
+
+    sc, hw = host_world("c1")
+    clip, heights = scenes.scene_clipmap(sc, 0)
+    world = O.World.from_host(hw, clip, heights)
+    rng = np.random.default_rng(17)
+    poses = [vx.make_pose((64.0, 12.8, 64.0), yaw, pitch, 70.0)
+             for yaw in (0.0, math.pi / 2, math.pi / 4, 1.1) for pitch in (0.0, -0.3, -math.pi / 4)]
+    poses.append(vx.make_pose((10.0, scenes.surface_m(sc.params, 10.0, 10.0) + 3.0, 10.0), math.pi / 4,
+                              math.radians(-2.0), 70.0))
+    poses.append(vx.make_pose((-30.0, 40.0, -25.0), 0.7, -0.35, 70.0))      # outside the grid, looking in
+    for _ in range(6):
+        poses.append(vx.make_pose(tuple(rng.uniform((0, 2, 0), (102.4, 30, 102.4))), float(rng.uniform(0, 6.3)),
+                                  float(rng.uniform(-1.2, 0.3)), 70.0))
+    res = (160, 96)
+    over24 = 0
+    for i, pose in enumerate(poses):
+        cam = vx.camera_build(pose, res, res, 0, grid_origin=sc.grid_origin, grid_dims=sc.grid_dims,
+                              chunk_count=hw.chunk_count)
+        a = vxd_cpu.tile_lists(world, cam, *res, wave=False)
+        b = vxd_cpu.tile_lists(world, cam, *res, wave=True)
+        for key in ("count", "cached", "slots"):
+            assert np.array_equal(a[key], b[key]), (i, key)
+        assert np.array_equal(a["enter"].view(np.uint32), b["enter"].view(np.uint32)), i
+        assert b["overflow"] == 0
+        raw = O.tile_raw_counts(world, cam, *res)
+        assert np.array_equal(np.minimum(raw.reshape(-1), 192), np.minimum(a["count"], 192)), i
+        over24 += int((a["count"] > 24).sum())
+    assert over24 > 20

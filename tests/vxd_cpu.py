@@ -45,6 +45,8 @@ def lib():
         _LIB.vxd_cpu_primary.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,
                                          vp, vp]
         _LIB.vxd_cpu_primary.restype = C.c_int
+        _LIB.vxd_cpu_tile_lists.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int, vp, vp, vp, vp, vp]
+        _LIB.vxd_cpu_tile_lists.restype = C.c_int
         _LIB.vxd_cpu_query.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_int]
         _LIB.vxd_cpu_query.restype = C.c_int
         _LIB.vxd_cpu_convert.argtypes = [vp, C.c_uint32, vp, vp]
@@ -124,3 +126,16 @@ def fast_divisors() -> list:
     lib().vxd_cpu_fast_divisor.argtypes = [C.c_uint32]
     lib().vxd_cpu_fast_divisor_count.restype = C.c_uint32
     return [float(lib().vxd_cpu_fast_divisor(k)) for k in range(lib().vxd_cpu_fast_divisor_count())]
+
+
+def tile_lists(world, cam, w: int, h: int, wave: bool) -> dict:
+    """The tile candidate lists of a w x h view: the one-lane gather (wave=False)
+    or the wavefront phases of the product kernel (wave=True)."""
+    nt = ((w + 7) // 8) * ((h + 7) // 8)
+    counts, cached = np.zeros(nt, np.uint32), np.zeros(nt, np.uint32)
+    slots, enter = np.zeros((nt, 24), np.uint32), np.zeros((nt, 24), np.float32)
+    over = np.zeros(1, np.uint32)
+    assert lib().vxd_cpu_tile_lists(C.addressof(world.scene), C.addressof(cam), w, h, 1 if wave else 0,
+                                    counts.ctypes.data, cached.ctypes.data, slots.ctypes.data, enter.ctypes.data,
+                                    over.ctypes.data) == 0
+    return {"count": counts, "cached": cached, "slots": slots, "enter": enter, "overflow": int(over[0])}

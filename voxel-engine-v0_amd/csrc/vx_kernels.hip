@@ -239,7 +239,23 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
       const bool prof = PROF && (V.cam->profile_flags & 1u) != 0u;
       C.prof = prof ? L.prof : nullptr;
       __syncthreads();  // the previous tile's lanes are done with L
-      if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
+      // The tile's candidate chunk list over the whole wavefront (the phases
+      // of plain::gather_*), or on one lane when a probe ray visits more cells
+      // than the phases hold.  The workgroup is one wavefront, and every
+      // branch below is uniform over it.
+      plain::gather_walk(C, L, wgx, wgy, threadIdx.x);
+      __syncthreads();
+      if (plain::gather_overflow(L)) {
+        if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
+      } else {
+        plain::gather_load(C, L, threadIdx.x);
+        __syncthreads();
+        plain::gather_first(L, threadIdx.x);
+        __syncthreads();
+        plain::gather_place(L, threadIdx.x);
+        __syncthreads();
+        if (threadIdx.x == 0) plain::gather_finish(L);
+      }
       __syncthreads();
       const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
       if (gx < fb.rw && gy < fb.rh) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
