@@ -19,6 +19,7 @@ sample of the same frame on the host cores.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -68,38 +69,68 @@ def parse():
 
 
 TRAFFIC_FILE = REPO / "profiles" / "k_primary_batch_traffic.md"
+PROFILE_FILE = REPO / "profiles" / "k_primary_batch_4k_flight.md"
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock: 39.3 T non-packed lane-ops/s
+
+
+def kernel_build() -> str:
+    """A digest of the kernel sources, the public header and the build flags: a
+    profile's figures describe this build only if it carries the same digest."""
+    h = hashlib.sha256()
+    pkg = REPO / "voxel-engine-v0_amd"
+    for path in sorted((pkg / "csrc").iterdir()) + [pkg / "build.py", REPO / "include" / "voxhip.h"]:
+        h.update(path.name.encode())
+        h.update(path.read_bytes())
+    return h.hexdigest()[:16]
+
+
+def profile_keys(path: Path) -> dict:
+    """The `key: value` lines of a committed profile summary."""
+    out = {}
+    if path.exists():
+        for line in path.read_text().splitlines():
+            key, sep, val = line.partition(":")
+            if sep and key and " " not in key.strip() and not key.startswith(("|", "#", "*", "`")):
+                out[key.strip()] = val.strip()
+    return out
+
+
+def profile_of_this_build(path: Path):
+    """The profile's keys if it was made with this kernel build, else (None, reason)."""
+    keys = profile_keys(path)
+    if not keys:
+        return None, f"{path.relative_to(REPO)} is missing"
+    if keys.get("kernel_build") != kernel_build():
+        return None, (f"{path.relative_to(REPO)} describes kernel build {keys.get('kernel_build')}, "
+                      f"not this one ({kernel_build()})")
+    return keys, None
 
 
 def measured_traffic(W, H):
     """HBM bytes per k_primary_batch launch at this resolution, from the committed
-    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
-    if not TRAFFIC_FILE.exists():
-        return None
-    for line in TRAFFIC_FILE.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            val, _, res = line.split(":", 1)[1].partition("@")
-            if res.strip() == f"{W}x{H}":
-                return int(val.strip())
-    return None
-
-
-PROFILE_FILE = REPO / "profiles" / "k_primary_batch_4k_flight.md"
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of this kernel
+    build, or (None, reason)."""
+    keys, why = profile_of_this_build(TRAFFIC_FILE)
+    if keys is None:
+        return None, why
+    val, _, res = keys.get("traffic_bytes_per_launch", "").partition("@")
+    if res.strip() != f"{W}x{H}":
+        return None, f"no traffic figure at {W}x{H}"
+    return int(val.strip()), None
 
 
 def profiled_valu():
-    """VALU busy and lane utilisation of the primary kernel from the committed
-    rocprofv3 summary (SQ counters of the same 4K workload), or None."""
-    if not PROFILE_FILE.exists():
-        return None
-    out = {}
-    for line in PROFILE_FILE.read_text().splitlines():
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 2 and cells[0] in ("VALU busy", "VALU lane utilisation", "L2 hit rate"):
-            out[cells[0]] = round(float(cells[1].rstrip(" %")) / 100.0, 4)
-    if len(out) < 3:
-        return None
-    return {"valu_busy": out["VALU busy"], "valu_lane_utilisation": out["VALU lane utilisation"],
-            "l2_hit_rate": out["L2 hit rate"], "source": str(PROFILE_FILE.relative_to(REPO))}
+    """SQ/TCC counter figures of k_primary_batch from the committed rocprofv3
+    summary of the same 4K workload and this kernel build, or (None, reason)."""
+    keys, why = profile_of_this_build(PROFILE_FILE)
+    if keys is None:
+        return None, why
+    try:
+        return {"valu_insts_per_launch": float(keys["valu_insts_per_launch"]),
+                "valu_busy": float(keys["valu_busy"]), "valu_lane_utilisation": float(keys["valu_lane_utilisation"]),
+                "l2_hit_rate": float(keys["l2_hit_rate"]), "source": str(PROFILE_FILE.relative_to(REPO))}, None
+    except (KeyError, ValueError) as e:
+        return None, f"{PROFILE_FILE.relative_to(REPO)}: {e}"
 
 
 def copy_calibration(mib):
@@ -206,41 +237,80 @@ def main():
     prim_ms = float(np.mean(pass_ms[:, 0]))
     value = rays * a.steps / total_s
 
-    roofline = None
+    roofline = roofline_valu = None
     counts = None
     passes = None
     if not a.no_count:
-        # The counting forms of the passes (the reference algorithm's loads per
-        # ray), over the first three timed frames: primary, then godray.
+        # The counting forms of the passes over the first three timed frames:
+        # primary (the reference algorithm's loads, then the loads the product
+        # kernel executes), then godray.
         view.reset()
         tot = {n: 0 for n in vx.COUNT_NAMES}
+        tot_x = {n: 0 for n in vx.COUNT_NAMES}
         tot_g = {n: 0 for n in vx.COUNT_NAMES}
+        parts = {n: 0 for n in vx.COUNT_PARTS}
+        parts_x = {n: 0 for n in vx.COUNT_PARTS}
         ncount = min(3, a.steps)
         ray_b = []
         view.counters(reset=True)
         for k in range(ncount):
             view.write_camera(cams[a.warmup + k])
             view.render(vx.PASS_PRIMARY | vx.PASS_COUNT)
+            pr = view.count_parts()
             c = view.counters(reset=True)
             ray_b.append(view.read(vx.OUT_RAY_BYTES).ravel())
+            view.render(vx.PASS_PRIMARY | vx.PASS_COUNT | vx.PASS_COUNT_EXECUTED)
+            px = view.count_parts()
+            cx = view.counters(reset=True)
             view.render(vx.PASS_GODRAY | vx.PASS_COUNT | vx.PASS_END_FRAME)
             cg = view.counters(reset=True)
             for n in tot:
                 tot[n] += c[n]
+                tot_x[n] += cx[n]
                 tot_g[n] += cg[n]
+            for n in parts:
+                parts[n] += pr[n]
+                parts_x[n] += px[n]
         counts = {n: v / ncount for n, v in tot.items()}
         bytes_per_launch = vx.algorithmic_bytes(counts)
+        exec_per_launch = vx.algorithmic_bytes({n: v / ncount for n, v in tot_x.items()})
         rb = np.concatenate(ray_b).astype(np.float64)
         achieved = bytes_per_launch / (prim_ms * 1e-3) / 1e9
+        achieved_x = exec_per_launch / (prim_ms * 1e-3) / 1e9
+        traffic, traffic_note = measured_traffic(W, H)
+        prof, prof_note = profiled_valu()
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(W, H),
-                    "kernel": "k_primary_batch", "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                    "kernel": "k_primary_batch", "kernel_build": kernel_build(),
+                    "algorithmic_bytes_per_launch": int(bytes_per_launch),
                     "algorithmic_bytes_per_ray": {"mean": round(bytes_per_launch / rays, 1),
                                                   "p50": round(float(np.percentile(rb, 50)), 1),
                                                   "p99": round(float(np.percentile(rb, 99)), 1),
                                                   "max": round(float(rb.max()), 1),
                                                   "sum_check": round(float(rb.sum()) / ncount / max(bytes_per_launch, 1), 6)},
-                    "profiled": profiled_valu()}
+                    "bytes_per_ray_by_part": {n: round(v / ncount / rays, 1) for n, v in parts.items()},
+                    "executed": {"note": "the loads the product kernel makes (its exact work skips on)",
+                                 "bytes_per_launch": int(exec_per_launch),
+                                 "bytes_per_ray": round(exec_per_launch / rays, 1),
+                                 "bytes_per_ray_by_part": {n: round(v / ncount / rays, 1) for n, v in parts_x.items()},
+                                 "achieved": round(achieved_x, 2), "frac": round(achieved_x / HBM_PEAK_GBS, 5)},
+                    "profiled": prof}
+        if traffic is None:
+            roofline["traffic_note"] = traffic_note
+        if prof is None:
+            roofline["profiled_note"] = prof_note
+        else:
+            # The VALU roof: lane-operations the kernel executes (VALU instructions
+            # x 64 lanes x the measured lane utilisation) per second of this run's
+            # primary pass, against 39.3 T non-packed lane-ops/s.
+            lane_ops = prof["valu_insts_per_launch"] * 64.0 * prof["valu_lane_utilisation"]
+            issue_ops = prof["valu_insts_per_launch"] * 64.0
+            roofline_valu = {"bound": "valu", "achieved": round(lane_ops / (prim_ms * 1e-3) / 1e12, 3),
+                             "peak": round(VALU_PEAK_LANE_OPS / 1e12, 3), "unit": "T lane-op/s",
+                             "frac": round(lane_ops / (prim_ms * 1e-3) / VALU_PEAK_LANE_OPS, 4),
+                             "issue_frac": round(issue_ops / (prim_ms * 1e-3) / VALU_PEAK_LANE_OPS, 4),
+                             "lane_ops_per_ray": round(lane_ops / rays, 1),
+                             "source": prof["source"]}
         # The other passes against the HBM roof: godray's counted loads, and the
         # streaming passes' fixed bytes per pixel (local TAA 48, composite 212,
         # composite TAA 64).
@@ -296,6 +366,7 @@ def main():
                          zip(("primary", "local_taa", "godray", "composite", "composite_taa"), pass_ms.mean(axis=0))},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "roofline": roofline,
+        "roofline_valu": roofline_valu,
         "pass_roofline_hbm": passes,
         "cpu_baseline": cpu,
         "config0_primary_640x360": c0,

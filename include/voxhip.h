@@ -245,6 +245,10 @@ int vx_write_camera(vx_view* view, const vx_camera* cam);
 #define VX_PASS_FRAME          0x9Fu   /* all five passes + end of frame */
 #define VX_PASS_WRITE_HITS     0x100u  /* also write the vx_hit buffer in primary */
 #define VX_PASS_COUNT          0x200u  /* run the load-counting variant of primary/godray */
+/* With VX_PASS_COUNT: the primary pass counts the loads the product kernel
+ * executes (its exact work skips on: light gathers with no light in reach,
+ * air regions of the light rays) instead of the reference algorithm's. */
+#define VX_PASS_COUNT_EXECUTED 0x400u
 int vx_render(vx_view* view, uint32_t pass_mask, void* stream);
 
 /* Output selectors for vx_read_output / vx_output_device_ptr.  Linear row-major
@@ -275,6 +279,13 @@ int vx_write_history(vx_view* view, int which, const void* src, size_t bytes);
  * reset: elements loaded from [grid, meta, macro, node, rope, colinfo, clip]
  * and texture bytes [read, written]. */
 int vx_view_counters(vx_view* view, uint64_t out[9], int reset);
+/* The same counted loads' bytes by part of the primary pass (no reference
+ * counterpart): [tile candidate gather, voxel traversal, grass (column probes
+ * and blade refinement), sun shadow, voxel shading, local-light gather,
+ * heightfield, textures (history reads and output writes)].  Accumulated with
+ * vx_view_counters' counts and cleared by its reset. */
+#define VX_COUNT_PARTS 8
+int vx_view_count_parts(vx_view* view, uint64_t out[VX_COUNT_PARTS]);
 
 /* The primary pass's profile counters (ray_main.wgsl:32-74), raised while the
  * camera's profile_flags & 1 by every primary pass of this view (vx_render

@@ -113,7 +113,8 @@ extern "C" {
 // mode 0: the one-kernel pass (k_primary<false>), which adds the profile
 // counters into prof[5] when cam->profile_flags & 1 (as k_primary_batch does
 // through each tile's LDS counters); 2: the load-counting kernel
-// (k_primary<true>), which also adds its counts into counts[9].
+// (k_primary_count_reference), which also adds its counts into counts[9] and
+// its part byte totals into counts[9..17).
 int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, uint32_t h, const float* primary_hist_in,
                     const float* shadow_in, float* color, float* depth, float* local, float* primary_hist_out,
                     float* shadow_out, vx_hit* hits, int mode, int threads, uint64_t* counts, uint64_t* prof) {
@@ -189,6 +190,7 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
     for (const Counts& c : cnt) {
       const uint32_t v[kNumCounts] = {c.grid, c.meta, c.macro, c.node, c.rope, c.colinfo, c.clip, c.tex_rd, c.tex_wr};
       for (int i = 0; i < kNumCounts; ++i) counts[i] += v[i];
+      for (int i = 0; i < (int)kNumParts; ++i) counts[kNumCounts + i] += c.part_bytes[i];
     }
   }
   return 0;

@@ -55,7 +55,10 @@ def test_device_code_matches_oracle(vx, name, res, frame_ids):
             assert np.array_equal(fr.shadow.view(np.uint32), ref.shadow.view(np.uint32)), mode
             assert np.array_equal(fr.local.view(np.uint32), ref.local.view(np.uint32)), mode
             if mode == "counted":
-                assert [int(c) for c in counts] == [oc.as_dict()[n] for n in vx.COUNT_NAMES]
+                assert [int(c) for c in counts[:9]] == [oc.as_dict()[n] for n in vx.COUNT_NAMES]
+                # the bytes by part of the pass add up to the counted bytes
+                assert int(counts[9:].sum()) == vx.algorithmic_bytes(dict(zip(vx.COUNT_NAMES, counts[:9].tolist())))
+                assert counts[9 + 1] > 0 and counts[9 + 7] > 0   # traversal and textures
         assert int(np.sum(ref.hits["kind"] == 1)) > 0
         # The lit world exercises the light gather's full march (the others its
         # no-light shortcut): some pixels then receive local light.

@@ -65,7 +65,7 @@ def primary(world, cam, fr, hits_dtype, mode: str = "one", threads: int = 0, pro
     hist_out = np.zeros_like(fr.primary_hist)
     shadow_out = np.zeros_like(fr.shadow)
     hits = np.zeros((fr.rh, fr.rw), hits_dtype)
-    counts = np.zeros(9, np.uint64)
+    counts = np.zeros(9 + 8, np.uint64)
     rc = lib().vxd_cpu_primary(C.addressof(world.scene), C.addressof(cam), fr.rw, fr.rh, fr.primary_hist.ctypes.data,
                                fr.shadow.ctypes.data, fr.color.ctypes.data, fr.depth.ctypes.data, fr.local.ctypes.data,
                                hist_out.ctypes.data, shadow_out.ctypes.data, hits.ctypes.data, MODES[mode], threads,

@@ -28,6 +28,11 @@ namespace counted {
 #include "vxd_passes.inc"
 #undef VXD_COUNT
 }  // namespace counted
+namespace executed {
+#define VXD_COUNT 2
+#include "vxd_passes.inc"
+#undef VXD_COUNT
+}  // namespace executed
 }  // namespace vxd
 
 namespace {
@@ -50,13 +55,15 @@ __device__ __forceinline__ uint32_t count_bytes(const Counts& c) {
          c.tex_rd + c.tex_wr;
 }
 
+// Adds the wavefront's counts into out: the kNumCounts array counts, then
+// the kNumParts part byte totals.
 template <bool COUNT>
 __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long* out) {
   if (!COUNT) return;
   const uint32_t v[kNumCounts] = {c.grid, c.meta, c.macro, c.node, c.rope, c.colinfo, c.clip, c.tex_rd, c.tex_wr};
 #pragma unroll
-  for (int i = 0; i < kNumCounts; ++i) {
-    unsigned long long s = v[i];
+  for (int i = 0; i < kNumCounts + (int)kNumParts; ++i) {
+    unsigned long long s = i < kNumCounts ? v[i] : c.part_bytes[i - kNumCounts];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63u) == 0u) atomicAdd(out + i, s);
   }
@@ -74,45 +81,40 @@ __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long
 #define VXD_GODRAY_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6, 8)))
 
 // ---------------------------------------------------------------------------
-// Primary pass, one kernel: one 8x8 tile per wavefront runs the whole pass per
-// pixel.  COUNT adds per-lane counters of the reference algorithm's loads.
+// Primary pass, counting forms: one 8x8 tile per wavefront runs the whole
+// pass per pixel with per-lane load counters.  k_primary_count_reference
+// counts the reference algorithm's loads (the product's exact work skips
+// off); k_primary_count_executed counts the loads the product kernel makes.
+// ray_bytes receives each ray's bytes, the tile's gather shared over its 64 rays.
 // ---------------------------------------------------------------------------
-template <bool COUNT>
-__global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary(
-    Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, uint32_t tiles_x, uint32_t ntiles,
-    unsigned long long* __restrict__ counters, float* __restrict__ ray_bytes) {
-  __shared__ TileLDS L;
-  const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
-  const uint32_t wgx = tile % tiles_x, wgy = tile / tiles_x;
-  if (threadIdx.x < 12u) L.grass_taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
-  const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
-  if constexpr (COUNT) {
-    counted::Ctx C;
-    C.S = S;
-    C.cam = cam;
-    C.grass_taper = L.grass_taper;
-    C.grass_blades = L.grass_blades + threadIdx.x;
-    Counts cnt = {};
-    C.cnt = &cnt;
-    if (threadIdx.x == 0) counted::tile_build(C, L, wgx, wgy);
-    // Lane 0's counts so far are the tile's candidate gather, shared by its 64 rays.
-    const uint32_t tile_b = threadIdx.x == 0 ? count_bytes(cnt) : 0u;
-    __syncthreads();
-    if (gx < fb.rw && gy < fb.rh) counted::primary_pixel(C, L, fb, gx, gy, counted::sky_bg(F3(0.0f, 1.0f, 0.0f)));
-    const float share = (float)__shfl(tile_b, 0, 64) * (1.0f / 64.0f);
-    if (ray_bytes && gx < fb.rw && gy < fb.rh) ray_bytes[(size_t)gy * fb.rw + gx] = (float)(count_bytes(cnt) - tile_b) + share;
-    flush_counts<true>(cnt, counters);
-  } else {
-    plain::Ctx C;
-    C.S = S;
-    C.cam = cam;
-    C.grass_taper = L.grass_taper;
-    C.grass_blades = L.grass_blades + threadIdx.x;
-    if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
-    __syncthreads();
-    if (gx < fb.rw && gy < fb.rh) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+#define VXD_COUNTING_PRIMARY(NS, NAME)                                                                                  \
+  __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void NAME(                                                    \
+      Scene S, const vx_camera* __restrict__ cam, FrameBuffers fb, uint32_t tiles_x, uint32_t ntiles,                 \
+      unsigned long long* __restrict__ counters, float* __restrict__ ray_bytes) {                                      \
+    __shared__ TileLDS L;                                                                                              \
+    const uint32_t tile = xcd_tile(blockIdx.x, ntiles);                                                                \
+    const uint32_t wgx = tile % tiles_x, wgy = tile / tiles_x;                                                         \
+    if (threadIdx.x < 12u) L.grass_taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);                        \
+    const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);                             \
+    NS::Ctx C;                                                                                                         \
+    C.S = S;                                                                                                           \
+    C.cam = cam;                                                                                                       \
+    C.grass_taper = L.grass_taper;                                                                                     \
+    C.grass_blades = L.grass_blades + threadIdx.x;                                                                     \
+    Counts cnt = {};                                                                                                   \
+    C.cnt = &cnt;                                                                                                      \
+    if (threadIdx.x == 0) NS::tile_build(C, L, wgx, wgy);                                                              \
+    const uint32_t tile_b = threadIdx.x == 0 ? count_bytes(cnt) : 0u; /* lane 0: the tile's gather so far */          \
+    __syncthreads();                                                                                                   \
+    if (gx < fb.rw && gy < fb.rh) NS::primary_pixel(C, L, fb, gx, gy, NS::sky_bg(F3(0.0f, 1.0f, 0.0f)));               \
+    const float share = (float)__shfl(tile_b, 0, 64) * (1.0f / 64.0f);                                                 \
+    if (ray_bytes && gx < fb.rw && gy < fb.rh)                                                                         \
+      ray_bytes[(size_t)gy * fb.rw + gx] = (float)(count_bytes(cnt) - tile_b) + share;                                 \
+    flush_counts<true>(cnt, counters);                                                                                 \
   }
-}
+VXD_COUNTING_PRIMARY(counted, k_primary_count_reference)
+VXD_COUNTING_PRIMARY(executed, k_primary_count_executed)
+#undef VXD_COUNTING_PRIMARY
 
 __global__ __launch_bounds__(256) void k_local_taa(FrameBuffers fb) {
   const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u), y = blockIdx.y * 16u + (threadIdx.x >> 4);
@@ -130,6 +132,7 @@ __global__ __launch_bounds__(64) VXD_GODRAY_OCCUPANCY void k_godray(Scene S, con
     C.cam = cam;
     C.grass_taper = nullptr;  // the godray pass has no grass geometry
     Counts cnt = {};
+    cnt.part = kPartShadow;  // the godray pass's scene loads are its sun-shadow traces
     C.cnt = &cnt;
     if (gx < fb.rw && gy < fb.rh) counted::godray_pixel(C, fb, gx, gy);
     flush_counts<true>(cnt, counters);
@@ -391,11 +394,13 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_query(
 // Launchers (host)
 // ---------------------------------------------------------------------------
 
-hipError_t vx_launch_primary(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
-                             unsigned long long* counters, float* ray_bytes, hipStream_t st) {
+hipError_t vx_launch_primary_count(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb,
+                                   bool executed, unsigned long long* counters, float* ray_bytes, hipStream_t st) {
   const uint32_t tx = (fb.rw + 7u) / 8u, ty = (fb.rh + 7u) / 8u, nt = tx * ty;
-  if (count) hipLaunchKernelGGL(k_primary<true>, dim3(nt), dim3(64), 0, st, S, cam_dev, fb, tx, nt, counters, ray_bytes);
-  else hipLaunchKernelGGL(k_primary<false>, dim3(nt), dim3(64), 0, st, S, cam_dev, fb, tx, nt, counters, nullptr);
+  if (executed)
+    hipLaunchKernelGGL(k_primary_count_executed, dim3(nt), dim3(64), 0, st, S, cam_dev, fb, tx, nt, counters, ray_bytes);
+  else
+    hipLaunchKernelGGL(k_primary_count_reference, dim3(nt), dim3(64), 0, st, S, cam_dev, fb, tx, nt, counters, ray_bytes);
   return hipGetLastError();
 }
 hipError_t vx_launch_local_taa(const vxd::FrameBuffers& fb, hipStream_t st) {

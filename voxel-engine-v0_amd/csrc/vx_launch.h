@@ -5,10 +5,13 @@
 
 #include "vxd_scene.h"
 
-// The primary pass as one kernel per 8x8 tile (count: the load-counting form,
-// which also writes each ray's counted bytes to ray_bytes when it is not null).
-hipError_t vx_launch_primary(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
-                             unsigned long long* counters, float* ray_bytes, hipStream_t st);
+// The primary pass's load-counting form, one kernel per 8x8 tile: the
+// reference algorithm's loads, or (executed) the loads the product kernel
+// makes with its exact work skips.  Adds the array counts and the part byte
+// totals into counters and writes each ray's counted bytes to ray_bytes when
+// it is not null.
+hipError_t vx_launch_primary_count(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb,
+                                   bool executed, unsigned long long* counters, float* ray_bytes, hipStream_t st);
 hipError_t vx_launch_local_taa(const vxd::FrameBuffers& fb, hipStream_t st);
 hipError_t vx_launch_godray(const vxd::Scene& S, const vx_camera* cam_dev, const vxd::FrameBuffers& fb, bool count,
                             unsigned long long* counters, hipStream_t st);

@@ -149,6 +149,11 @@ struct vx_view {
 
 namespace {
 
+// A view's counter words: the array counts, then the part byte totals.
+constexpr size_t kViewCounters = 32;
+static_assert(vxd::kNumCounts + vxd::kNumParts <= kViewCounters, "view counters hold the counts and the parts");
+static_assert(vxd::kNumParts == VX_COUNT_PARTS, "the ABI's part list is the kernels'");
+
 struct DeviceGuard {
   int prev = 0;
   explicit DeviceGuard(int dev) {
@@ -497,7 +502,7 @@ int vx_view_create(vx_ctx* ctx, uint32_t pw, uint32_t ph, uint32_t rw, uint32_t 
   int rc = dalloc(&v->cam_dev, 1);
   if (!rc) rc = dalloc(&v->prof, 5);
   if (!rc) rc = alloc_view_buffers(v);
-  if (!rc) rc = dalloc(&v->counters, 16);
+  if (!rc) rc = dalloc(&v->counters, kViewCounters);
   for (int i = 0; i < 10 && !rc; ++i)
     if (hipEventCreate(&v->ev[i]) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "vx_view_create: hipEventCreate");
   if (rc) {
@@ -758,7 +763,8 @@ int vx_render(vx_view* v, uint32_t mask, void* stream) {
         const int rc = dalloc(&v->ray_bytes, (size_t)v->rw * v->rh);
         if (rc) return rc;
       }
-      VX_HIP(vx_launch_primary(S, v->cam_dev, fb, true, v->counters, v->ray_bytes, st));
+      VX_HIP(vx_launch_primary_count(S, v->cam_dev, fb, (mask & VX_PASS_COUNT_EXECUTED) != 0u, v->counters,
+                                     v->ray_bytes, st));
     } else {
       VX_HIP(vx_launch_primary_ws(S, B1, persistent, (v->profile_flags & 1u) != 0u, st));
     }
@@ -852,13 +858,23 @@ int vx_view_counters(vx_view* v, uint64_t out[9], int reset) {
   if (!v || !out) return vx_fail(VX_E_INVALID, "vx_view_counters: null argument");
   DeviceGuard g(v->ctx->device);
   VX_HIP(hipDeviceSynchronize());
-  unsigned long long tmp[16];
+  unsigned long long tmp[kViewCounters];
   VX_HIP(hipMemcpy(tmp, v->counters, sizeof(tmp), hipMemcpyDeviceToHost));
   for (int i = 0; i < 9; ++i) out[i] = tmp[i];
   if (reset) {
     VX_HIP(hipMemset(v->counters, 0, sizeof(tmp)));
     VX_HIP(hipStreamSynchronize(nullptr));  // before the view's non-blocking stream counts again
   }
+  return VX_OK;
+}
+
+int vx_view_count_parts(vx_view* v, uint64_t out[VX_COUNT_PARTS]) {
+  if (!v || !out) return vx_fail(VX_E_INVALID, "vx_view_count_parts: null argument");
+  DeviceGuard g(v->ctx->device);
+  VX_HIP(hipDeviceSynchronize());
+  unsigned long long tmp[kViewCounters];
+  VX_HIP(hipMemcpy(tmp, v->counters, sizeof(tmp), hipMemcpyDeviceToHost));
+  for (int i = 0; i < VX_COUNT_PARTS; ++i) out[i] = tmp[vxd::kNumCounts + i];
   return VX_OK;
 }
 

@@ -24,9 +24,21 @@ struct Scene {
   const uint32_t* __restrict__ chunk_light;
 };
 
-// Algorithmic loads by array (element counts; bytes = count * element size).
+// Parts of the primary pass that loads are charged to (VX_COUNT_PARTS):
+// the tile's candidate gather, the voxel traversal, grass (column probes and
+// blade refinement), the sun shadow, voxel shading (ambient occlusion from the
+// macro words), the local-light gather, the heightfield (clipmap), and texture
+// traffic (histories read, outputs written).
+enum CountPart : uint32_t {
+  kPartGather = 0, kPartTrace, kPartGrass, kPartShadow, kPartShade, kPartLight, kPartHeightfield, kPartTextures,
+  kNumParts
+};
+// Algorithmic loads by array (element counts; bytes = count * element size),
+// and the same loads' bytes by part of the pass.
 struct Counts {
   uint32_t grid, meta, macro, node, rope, colinfo, clip, tex_rd, tex_wr;
+  uint32_t part_bytes[kNumParts];
+  uint32_t part;  // the part loads are charged to now
 };
 constexpr int kNumCounts = 9;
 

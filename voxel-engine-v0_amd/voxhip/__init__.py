@@ -41,6 +41,8 @@ PASS_END_FRAME = 0x80
 PASS_FRAME = 0x9F
 PASS_WRITE_HITS = 0x100
 PASS_COUNT = 0x200
+PASS_COUNT_EXECUTED = 0x400
+COUNT_PARTS = ("gather", "trace", "grass", "shadow", "shade", "light", "heightfield", "textures")
 NOALLOC_MAX_VIEWS = 64
 
 OUT_COLOR, OUT_DEPTH, OUT_LOCAL, OUT_PRIMARY_HIST, OUT_SHADOW_HIST = 0, 1, 2, 3, 4
@@ -208,6 +210,7 @@ def _load() -> C.CDLL:
         "vx_view_counters": ([vp, C.POINTER(u64), C.c_int], C.c_int),
         "vx_view_timings": ([vp, C.POINTER(f32)], C.c_int),
         "vx_view_profile": ([vp, C.POINTER(u32), C.c_int], C.c_int),
+        "vx_view_count_parts": ([vp, C.POINTER(u64)], C.c_int),
         "vx_render_views_noalloc": ([vp, C.POINTER(vp), u32, u32, vp], C.c_int),
         "vx_query_rays": ([vp, C.POINTER(Camera), vp, u32, vp, vp], C.c_int),
         "vx_query_timing": ([vp, C.POINTER(f32)], C.c_int),
@@ -261,7 +264,7 @@ LIB = _load()
 EXPORTED = (
     "vx_last_error", "vx_version", "vx_device_count", "vx_create", "vx_destroy", "vx_upload_regions", "vx_write_grid",
     "vx_write_clipmap", "vx_view_create", "vx_view_destroy", "vx_view_resize", "vx_view_reset", "vx_write_camera",
-    "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings", "vx_view_profile", "vx_render_views_noalloc",
+    "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings", "vx_view_profile", "vx_render_views_noalloc", "vx_view_count_parts",
     "vx_server_create", "vx_server_destroy", "vx_server_submit", "vx_server_poll", "vx_server_edit",
     "vx_server_wait_edits", "vx_server_drain", "vx_server_close_stream", "vx_server_get_stats",
     "vx_query_rays", "vx_query_timing", "vx_query_set_config", "vx_query_get_config", "vx_query_tune", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
@@ -622,6 +625,12 @@ class View:
         out = (C.c_uint64 * 9)()
         _check(LIB.vx_view_counters(self._h, out, 1 if reset else 0), "vx_view_counters")
         return dict(zip(COUNT_NAMES, (int(v) for v in out)))
+
+    def count_parts(self) -> dict:
+        """Counted bytes by part of the primary pass (vx_view_count_parts); cleared by counters(reset=True)."""
+        out = (C.c_uint64 * len(COUNT_PARTS))()
+        _check(LIB.vx_view_count_parts(self._h, out), "vx_view_count_parts")
+        return dict(zip(COUNT_PARTS, (int(v) for v in out)))
 
     def profile(self, reset: bool = True) -> dict:
         """The primary pass's profile counters (vx_view_profile)."""
