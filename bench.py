@@ -656,7 +656,7 @@ def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budge
     with a concurrent stream of edit_rate random single-voxel edits/s, and
     admission against a `budget`-byte device memory budget and, when
     target_ms > 0, against that latency target (the server also bounds each
-    launch to half the target's worth of rays).  device_build: edited chunks
+    launch to a third of the target by its launch-cost model).  device_build: edited chunks
     are rebuilt by the device chunk builder from HBM-resident material instead
     of on host worker threads."""
     import threading

@@ -572,13 +572,18 @@ int  vx_clipmap_fill_rects(const vx_world_params* p, const vx_clip_rect* rects, 
  *     view) and a ray batch n*64 bytes (rays + hits) of memory_budget; a
  *     request that does not fit is rejected with VX_E_BUSY, and so is a frame
  *     of a stream that already has max_queued_frames frames pending.
- *   * Latency target (latency_target_ms > 0): the server keeps a running
- *     estimate of its throughput (rays per ms of launch time).  A request is
- *     rejected with VX_E_BUSY when the estimated time to finish the launch in
- *     flight, the queued requests and itself exceeds the target, and each
- *     launch takes queued requests in arrival order only up to half the
- *     target's worth of rays (at least one request), so one launch never
- *     holds the next for long.
+ *   * Latency target (latency_target_ms > 0): the server fits launch time as
+ *     fixed + a * view pixels + b * query rays over its recent launches
+ *     (launches that create a view excluded).  Each launch takes queued
+ *     requests in arrival order while its estimated time stays within a third
+ *     of the target (at least one request), so one launch never holds the
+ *     next for long.  A request is rejected with VX_E_BUSY when margin *
+ *     (the rest of the launch in flight + the launches that the queue ahead
+ *     and the request need) exceeds the target, where margin is the 99th
+ *     percentile of latency / estimate over the last 256 requests admitted
+ *     near the threshold (1.1 until 32 have finished): p99 latency tracks the
+ *     target.  A server idle for 100 ms admits the next request regardless,
+ *     so the estimate cannot lock every request out.
  *   * Edits: single-voxel edits rebuild their chunk on host worker threads
  *     (generator fill + edits + vx_build_chunk); at the next launch boundary
  *     each rebuilt chunk is written to a fresh node-arena range (best-fit),
@@ -613,10 +618,12 @@ typedef struct vx_request {
     uint32_t width, height;          /* VIEW: render = present resolution */
     vx_camera camera;                /* VIEW: this frame's camera; RAYS: the scene camera (ray
                                         batches whose scene fields differ run as separate launches) */
-    const vx_ray* rays;              /* RAYS: host array, copied at submit */
+    const vx_ray* rays;              /* RAYS: host array, copied at submit (into the server's pinned
+                                        ray pool of budget/64 rays, 64 Ki..8 Mi, when it has room) */
     uint32_t n_rays, pad;
     float*   out_rgba;               /* VIEW: optional host buffer, width*height RGBA32F final frame */
-    vx_hit*  out_hits;               /* RAYS: optional host buffer of n_rays hits */
+    vx_hit*  out_hits;               /* RAYS: optional host buffer of n_rays hits, written before the
+                                        result is posted (ray batches post before the launch's views) */
     uint64_t tag;                    /* echoed in the result */
 } vx_request;
 typedef struct vx_result {

@@ -121,6 +121,32 @@ def test_admission_budget(gpu):
     srv.close()
 
 
+def test_ray_batches_through_pool_and_staging(gpu):
+    """A ray batch is copied at submit into the server's pinned pool (a
+    budget's worth of rays, at most 8 Mi); one larger than the pool goes
+    through the launch's own staging instead.  Both return the hits of a
+    direct vx_query_rays on the same rays, also when they share launches."""
+    from voxhip import scenes
+
+    sc, hw = host_world("c0")
+    clip, heights = scenes.scene_clipmap(sc)
+    ctx = _ctx(gpu, hw, clip, heights)
+    qcam = _scene_cam(gpu, sc, hw)
+    big = 9_000_000  # above the 8 Mi-ray pool: staged
+    rays = scenes.query_rays(np.random.default_rng(12), big, sc.grid_origin, sc.grid_dims, tmax_m=51.2)
+    direct = gpu.query_rays(ctx, qcam, rays)
+    srv = gpu.Server(ctx, hw, memory_budget=1 << 30)
+    spans = [(0, 5000), (0, big), (123_457, 200_000), (big - 70_000, big)]
+    hits = [np.zeros(b - a, gpu.HIT_DTYPE) for a, b in spans]
+    for i, (a, b) in enumerate(spans):
+        assert srv.submit_rays(qcam, rays[a:b], tag=i, out=hits[i])
+    results = _wait_all(srv, len(spans))
+    assert all(r.status == 0 for r in results)
+    for i, (a, b) in enumerate(spans):
+        assert np.array_equal(hits[i].view(np.uint8), direct[a:b].view(np.uint8)), i
+    srv.close()
+
+
 def test_close_stream_while_streams_open(gpu):
     """vx_server_close_stream waits for the stream's queued frames with the lock
     released; meanwhile another thread opens 64 new streams (the stream map

@@ -26,6 +26,7 @@
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -46,6 +47,11 @@ double now_ms() {
 
 constexpr uint64_t kViewBytesPerPx = 250;  // history + outputs + hits of one view (see vx_view_create)
 constexpr uint64_t kRayBytes = sizeof(vx_ray) + sizeof(vx_hit);
+// With a latency target, one launch takes at most this share of the target's
+// worth of rays at the measured rate: a request that arrives just after a
+// launch starts waits for it and then runs in the next, so two launches must
+// fit in the target.
+constexpr double kLaunchShare = 1.0 / 3.0;
 
 // Best-fit allocator over the node arena (element units).
 class Arena {
@@ -115,10 +121,112 @@ class Arena {
   std::multimap<uint64_t, uint64_t> by_size_;    // length -> offset
 };
 
+// Rays of one launch or queue, by kind: view pixels (primary, godray,
+// composite and TAA per pixel) and query rays (a primary trace only) cost
+// different amounts each.
+struct Work {
+  double view = 0.0, query = 0.0;
+  bool empty() const { return view <= 0.0 && query <= 0.0; }
+  Work& operator+=(const Work& o) {
+    view += o.view;
+    query += o.query;
+    return *this;
+  }
+  Work& operator-=(const Work& o) {
+    view -= o.view;
+    query -= o.query;
+    return *this;
+  }
+};
+
+// Launch time as fixed + per_view * view pixels + per_query * query rays,
+// fitted over recent launches by exponentially weighted least squares (weight
+// 0.9 per launch, about the last ten).  A launch's fixed costs (launch
+// boundary, camera writes, syncs) are not spread over its rays, and the two
+// kinds get their own rate.  While the launches seen so far cannot separate
+// the three terms (one kind only, or one mix), the missing terms fall back to
+// the fit on all rays (fixed + per_ray * rays), and that to proportional.
+class LaunchCost {
+ public:
+  void add(const Work& w, double ms) {
+    const double x[3] = {1.0, w.view, w.query};
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) a_[i][j] = kDecay * a_[i][j] + x[i] * x[j];
+      b_[i] = kDecay * b_[i] + x[i] * ms;
+    }
+    fit();
+  }
+  bool ready() const { return a_[0][0] > 0.5 && (a_[1][1] > 0.0 || a_[2][2] > 0.0); }
+  double fixed() const { return c_[0]; }
+  double ms(const Work& w) const { return c_[0] + c_[1] * w.view + c_[2] * w.query; }
+  // Time of `w` beyond the fixed cost.
+  double rays_ms(const Work& w) const { return c_[1] * w.view + c_[2] * w.query; }
+
+ private:
+  void fit() {
+    const double n = a_[0][0];
+    // All rays as one kind: fixed + per_ray * (view + query).
+    const double sx = a_[0][1] + a_[0][2], sy = b_[0];
+    const double sxx = a_[1][1] + 2.0 * a_[1][2] + a_[2][2], sxy = b_[1] + b_[2];
+    const double mx = sx / n, my = sy / n, var = sxx / n - mx * mx;
+    double per = var > 1e-6 * mx * mx ? (sxy / n - mx * my) / var : 0.0;
+    if (!(per > 0.0)) per = my / std::max(mx, 1.0);
+    double fixed = std::min(std::max(my - per * mx, 0.0), my);
+    c_[0] = fixed;
+    c_[1] = c_[2] = per;
+    // Three terms, when both kinds vary independently enough to separate
+    // them: the normal equations in variables scaled to unit mean square, so
+    // a small pivot means the launches seen are (nearly) collinear.
+    if (!(a_[1][1] > 0.0 && a_[2][2] > 0.0)) return;
+    const double sc[3] = {1.0, std::sqrt(n / a_[1][1]), std::sqrt(n / a_[2][2])};
+    double m[3][4];
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) m[i][j] = a_[i][j] / n * sc[i] * sc[j];
+      m[i][3] = b_[i] / n * sc[i];
+    }
+    for (int k = 0; k < 3; ++k) {  // Gauss-Jordan elimination with partial pivoting
+      int piv = k;
+      for (int i = k + 1; i < 3; ++i)
+        if (std::fabs(m[i][k]) > std::fabs(m[piv][k])) piv = i;
+      if (std::fabs(m[piv][k]) < 1e-4) return;  // not separable yet: keep the fit on all rays
+      for (int j = 0; j < 4; ++j) std::swap(m[k][j], m[piv][j]);
+      for (int i = 0; i < 3; ++i) {
+        if (i == k) continue;
+        const double f = m[i][k] / m[k][k];
+        for (int j = k; j < 4; ++j) m[i][j] -= f * m[k][j];
+      }
+    }
+    const double f3 = m[0][3] / m[0][0], v3 = m[1][3] / m[1][1] * sc[1], q3 = m[2][3] / m[2][2] * sc[2];
+    if (!(f3 >= 0.0 && v3 > 0.0 && q3 > 0.0)) return;
+    c_[0] = f3;
+    c_[1] = v3;
+    c_[2] = q3;
+  }
+  static constexpr double kDecay = 0.9;
+  double a_[3][3] = {}, b_[3] = {};
+  double c_[3] = {};  // fixed, per view pixel, per query ray
+};
+
+// Admission multiplies its estimate by a margin before comparing it with the
+// target: the 99th percentile of (latency / estimate) over the last kCalib
+// requests admitted near the threshold (estimate at least half the admissible
+// one), so that p99 latency holds the target whatever the model misses
+// (launches that later arrivals fill up, scenes that cost more per ray);
+// kAdmitMargin until kCalibMin such requests have finished.
+constexpr double kAdmitMargin = 1.1;
+constexpr size_t kCalib = 256, kCalibMin = 32;
+constexpr double kMarginMax = 4.0;
+// An idle server admits the next request unconditionally after this long without a launch.
+constexpr double kProbeAfterMs = 100.0;
+
+constexpr uint64_t kNoPin = ~0ull;
+
 struct Pending {
-  vx_request req;
-  std::vector<vx_ray> rays;  // copied at submit
+  vx_request req;            // req.rays is the caller's and not read after submit
+  uint64_t pin = kNoPin;     // RAYS: element offset of its block in the pinned ray/hit pool
+  std::vector<vx_ray> rays;  // RAYS without a pool block: copied at submit
   double submit_ms = 0.0;
+  double est_ms = -1.0;      // admission's latency estimate (without margin), -1 if admitted without one
   uint64_t reserved = 0;     // budget bytes released at completion
 };
 
@@ -169,11 +277,14 @@ struct vx_server {
   std::deque<vx_result> outbox;
   std::unordered_map<uint32_t, Stream> streams;
   uint64_t mem_in_use = 0;
-  // Latency target: throughput estimate (rays per ms of launch time, 0 until
-  // the first launch), rays admitted and not yet launched, and the launch in
-  // flight (its rays and start time).
-  double rate = 0.0;
-  uint64_t queued_rays = 0, inflight_rays = 0;
+  // Latency target: the launch-time model, rays admitted and not yet
+  // launched, and the launch in flight (its rays and start time).
+  LaunchCost cost;
+  Work queued, inflight;
+  double margin = kAdmitMargin;
+  std::vector<double> calib;  // ring of latency / estimate ratios
+  size_t calib_at = 0;
+  double last_launch_end_ms = 0.0;
   double inflight_t0 = 0.0;
   bool stop = false;
   std::thread device;
@@ -198,6 +309,13 @@ struct vx_server {
   uint64_t* d_off = nullptr;
   uint8_t* d_val = nullptr;
   size_t d_edit_cap = 0;
+  // Pinned pool for ray batches: a request's rays are copied into its block
+  // on the submitting thread and its hits come back into the block at the
+  // same offset, so a launch queues only DMA copies.  Requests that find no
+  // room go through the launch's own staging.
+  vx_ray* pin_rays = nullptr;
+  vx_hit* pin_hits = nullptr;
+  Arena pin;  // element offsets (guarded by mu)
 };
 
 namespace {
@@ -470,43 +588,85 @@ bool same_scene(const vx_camera& a, const vx_camera& b) {
 }
 
 uint64_t request_rays(const Pending& p) {
-  return p.req.kind == VX_REQ_VIEW ? (uint64_t)p.req.width * p.req.height : (uint64_t)p.rays.size();
+  return p.req.kind == VX_REQ_VIEW ? (uint64_t)p.req.width * p.req.height : (uint64_t)p.req.n_rays;
+}
+
+// Publishes a finished request (under mu): its result, latency, budget and pool block.
+void complete(vx_server* s, Pending& p, int status, double t0, double t1, std::vector<double>& lat) {
+  vx_result r{};
+  r.tag = p.req.tag;
+  r.kind = p.req.kind;
+  r.status = status;
+  r.submit_ms = p.submit_ms - s->t_start;
+  r.start_ms = t0 - s->t_start;
+  r.done_ms = t1 - s->t_start;
+  s->outbox.push_back(r);
+  lat.push_back(t1 - p.submit_ms);
+  if (p.est_ms > 0.0 && p.est_ms * s->margin >= 0.5 * s->cfg.latency_target_ms) {
+    const double ratio = (t1 - p.submit_ms) / p.est_ms;
+    if (s->calib.size() < kCalib)
+      s->calib.push_back(ratio);
+    else
+      s->calib[s->calib_at++ % kCalib] = ratio;
+  }
+  s->mem_in_use -= p.reserved;
+  s->st.rays += request_rays(p);
+  if (p.pin != kNoPin) {
+    s->pin.release(p.pin, p.req.n_rays);
+    p.pin = kNoPin;
+  }
+  ++s->st.completed;
+}
+
+Work request_work(const Pending& p) {
+  Work w;
+  (p.req.kind == VX_REQ_VIEW ? w.view : w.query) = (double)request_rays(p);
+  return w;
 }
 
 void device_main(vx_server* s) {
   hipSetDevice(vx_internal_device(s->ctx));
-  // Rays go up and hits come down through pinned staging, so the copies run on
-  // the DMA engines instead of as staging blits on the compute units.
+  // Device ray and hit buffers for a launch, and pinned staging for the
+  // requests that found no room in the pool (allocated when first needed):
+  // pinned host memory lets the copies run on the DMA engines instead of as
+  // staging blits on the compute units.  Device buffers hold 4 M rays from
+  // the start, doubled when a launch needs more: reallocating (a
+  // synchronising free) inside a launch would hold every request in it.
   vx_ray* d_rays = nullptr;
   vx_hit* d_hits = nullptr;
   size_t d_cap = 0;
   vx_ray* h_rays = nullptr;
   vx_hit* h_hits = nullptr;
   size_t h_cap = 0;
-  // Staging for 4 M rays from the start, doubled when a launch needs more:
-  // reallocating (a synchronising free, and pinning hundreds of MB of host
-  // pages) inside a launch would hold every request in it.
-  auto grow = [&](size_t nr) -> int {
-    if (nr <= h_cap && nr <= d_cap) return VX_OK;
-    const size_t cap = std::max({nr, 2 * std::max(h_cap, d_cap), (size_t)1 << 22});
-    if (h_rays) hipHostFree(h_rays);
-    if (h_hits) hipHostFree(h_hits);
+  auto grow_dev = [&](size_t nr) -> int {
+    if (nr <= d_cap) return VX_OK;
+    const size_t cap = std::max({nr, 2 * d_cap, (size_t)1 << 22});
     if (d_rays) hipFree(d_rays);
     if (d_hits) hipFree(d_hits);
-    h_rays = nullptr;
-    h_hits = nullptr;
     d_rays = nullptr;
     d_hits = nullptr;
-    h_cap = d_cap = 0;
-    if (hipHostMalloc((void**)&h_rays, cap * sizeof(vx_ray), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h_hits, cap * sizeof(vx_hit), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&d_rays, cap * sizeof(vx_ray)) != hipSuccess ||
+    d_cap = 0;
+    if (hipMalloc((void**)&d_rays, cap * sizeof(vx_ray)) != hipSuccess ||
         hipMalloc((void**)&d_hits, cap * sizeof(vx_hit)) != hipSuccess)
-      return vx_fail(VX_E_NOMEM, "server: ray staging for %zu rays", cap);
-    h_cap = d_cap = cap;
+      return vx_fail(VX_E_NOMEM, "server: device ray buffers for %zu rays", cap);
+    d_cap = cap;
     return VX_OK;
   };
-  if (grow(1) != VX_OK) {
+  auto grow_host = [&](size_t nr) -> int {
+    if (nr <= h_cap) return VX_OK;
+    const size_t cap = std::max(nr, 2 * h_cap);
+    if (h_rays) hipHostFree(h_rays);
+    if (h_hits) hipHostFree(h_hits);
+    h_rays = nullptr;
+    h_hits = nullptr;
+    h_cap = 0;
+    if (hipHostMalloc((void**)&h_rays, cap * sizeof(vx_ray), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_hits, cap * sizeof(vx_hit), hipHostMallocDefault) != hipSuccess)
+      return vx_fail(VX_E_NOMEM, "server: ray staging for %zu rays", cap);
+    h_cap = cap;
+    return VX_OK;
+  };
+  if (grow_dev(1) != VX_OK) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->last_error = VX_E_NOMEM;
   }
@@ -544,39 +704,47 @@ void device_main(vx_server* s) {
       s->dev_busy = !dev_slots.empty();
       // Queued requests join this launch in arrival order: at most one frame
       // per view stream, max_views_per_launch views and, with a latency
-      // target, half the target's worth of rays (at least one request); the
+      // target, a third of the target's worth of rays (at least one request); the
       // rest waits for the next launch.
-      const double cap = s->cfg.latency_target_ms > 0.0f && s->rate > 0.0 ? s->rate * 0.5 * s->cfg.latency_target_ms : 0.0;
-      uint64_t launch_rays = 0, query_rays = 0;
+      const double cap_ms = s->cfg.latency_target_ms > 0.0f && s->cost.ready()
+                                ? kLaunchShare * s->cfg.latency_target_ms
+                                : 0.0;
+      Work launch;
+      uint64_t query_rays = 0;
       std::deque<Pending> keep;
       std::vector<uint32_t> taken;
       while (!s->inbox.empty()) {
         Pending p = std::move(s->inbox.front());
         s->inbox.pop_front();
         const uint64_t n = request_rays(p);
-        const bool full = cap > 0.0 && launch_rays > 0 && (double)(launch_rays + n) > cap;
+        Work with = launch;
+        with += request_work(p);
+        const bool full = cap_ms > 0.0 && !launch.empty() && s->cost.ms(with) > cap_ms;
         if (p.req.kind == VX_REQ_VIEW) {
           const bool dup = std::find(taken.begin(), taken.end(), p.req.stream_id) != taken.end();
           if (full || dup || views.size() >= s->cfg.max_views_per_launch) {
             keep.push_back(std::move(p));
           } else {
             taken.push_back(p.req.stream_id);
-            launch_rays += n;
+            launch = with;
             views.push_back(std::move(p));
           }
-        } else if (full || (query_rays > 0 && query_rays + n > h_cap)) {  // ray batches beyond the staging wait
+        } else if (full || (query_rays > 0 && query_rays + n > d_cap)) {  // ray batches beyond the device buffers wait
           keep.push_back(std::move(p));
         } else {
-          launch_rays += n;
+          launch = with;
           query_rays += n;
           rays.push_back(std::move(p));
         }
       }
       s->inbox.swap(keep);
-      s->queued_rays -= launch_rays;
-      s->inflight_rays = launch_rays;
+      s->queued -= launch;
+      s->inflight = launch;
       s->inflight_t0 = now_ms();
     }
+    // The launch's time, for the requests' start and for the cost model,
+    // includes applying the edits at this boundary.
+    const double t0 = now_ms();
     int rc = builds.empty() ? VX_OK : apply_ready(s, builds);
     if (!rc && !dev_slots.empty()) {
       bool requeue = false;
@@ -595,16 +763,63 @@ void device_main(vx_server* s) {
       }
       s->dev_busy = false;
     }
-    const double t0 = now_ms();
-    // Views: bind each request to its stream's view and write the camera.
+    // Ray batches first: one fused query launch on the second stream.  Their
+    // upload is a copy kernel; issued after the views' persistent primary it
+    // would wait for that kernel's workgroups to drain.
+    size_t nr = 0, n_staged = 0;
+    for (Pending& p : rays) {
+      nr += p.req.n_rays;
+      if (p.pin == kNoPin) n_staged = nr;  // staging keeps the launch's offsets
+    }
+    if (!rc && nr) {
+      rc = grow_dev(nr);
+      if (!rc && n_staged) rc = grow_host(n_staged);
+      size_t at = 0;
+      for (Pending& p : rays) {
+        const size_t n = p.req.n_rays;
+        const vx_ray* src = p.pin != kNoPin ? s->pin_rays + p.pin : h_rays + at;
+        if (!rc && n && p.pin == kNoPin) std::memcpy(h_rays + at, p.rays.data(), n * sizeof(vx_ray));
+        if (!rc && n &&
+            hipMemcpyAsync(d_rays + at, src, n * sizeof(vx_ray), hipMemcpyHostToDevice, s->qstream) != hipSuccess)
+          rc = vx_fail(VX_E_DEVICE, "server: ray upload");
+        at += n;
+      }
+      // One fused launch per run of consecutive requests that share the scene
+      // fields of their camera (normally a single launch): a request is
+      // traced against the scene its own camera names.
+      for (size_t i = 0, r0 = 0; i < rays.size() && !rc; r0 += rays[i].req.n_rays, ++i) {
+        size_t j = i, n_run = rays[i].req.n_rays;
+        while (j + 1 < rays.size() && same_scene(rays[j + 1].req.camera, rays[i].req.camera))
+          n_run += rays[++j].req.n_rays;
+        if (n_run) rc = vx_query_rays(s->ctx, &rays[i].req.camera, d_rays + r0, (uint32_t)n_run, d_hits + r0, s->qstream);
+        for (; i < j; ++i) r0 += rays[i].req.n_rays;
+      }
+      // Hit downloads queue behind the query on its stream (only the ranges of
+      // requests that asked for their hits), overlapping the view passes.
+      at = 0;
+      for (Pending& p : rays) {
+        const size_t n = p.req.n_rays;
+        vx_hit* dst = p.pin != kNoPin ? s->pin_hits + p.pin : h_hits + at;
+        if (!rc && p.req.out_hits && n &&
+            hipMemcpyAsync(dst, d_hits + at, n * sizeof(vx_hit), hipMemcpyDeviceToHost, s->qstream) != hipSuccess)
+          rc = vx_fail(VX_E_DEVICE, "server: hit download");
+        at += n;
+      }
+    }
+    // Views: bind each request to its stream's view (created on its first
+    // frame) and write the camera.
     std::vector<vx_view*> vs;
+    bool created_view = false;  // the launch allocated a view: not a sample for the cost model
     for (Pending& p : views) {
       Stream* stp = nullptr;
       {
         std::lock_guard<std::mutex> lk(s->mu);
         stp = &s->streams[p.req.stream_id];
       }
-      if (!stp->view && !rc) rc = vx_view_create(s->ctx, p.req.width, p.req.height, p.req.width, p.req.height, &stp->view);
+      if (!stp->view && !rc) {
+        rc = vx_view_create(s->ctx, p.req.width, p.req.height, p.req.width, p.req.height, &stp->view);
+        created_view = true;
+      }
       if (!rc) rc = vx_write_camera(stp->view, &p.req.camera);
       vs.push_back(stp->view);
     }
@@ -620,84 +835,53 @@ void device_main(vx_server* s) {
       if (!rc) rc = vx_batch_render(batch, VX_PASS_FRAME, nullptr);
       rendered = true;
     }
-    // Ray batches: one fused query launch on the second stream.
-    size_t nr = 0;
-    for (Pending& p : rays) nr += p.rays.size();
-    if (!rc && nr) {
-      rc = grow(nr);
+    // Ray batches complete first: their hits are copied out to the callers
+    // while the views still render.
+    if (!rays.empty()) {
+      int rc_rays = rc;
+      if (!rc_rays && nr && hipStreamSynchronize(s->qstream) != hipSuccess)
+        rc_rays = vx_fail(VX_E_DEVICE, "server: query stream");
       size_t at = 0;
       for (Pending& p : rays) {
-        if (!rc && !p.rays.empty()) std::memcpy(h_rays + at, p.rays.data(), p.rays.size() * sizeof(vx_ray));
-        at += p.rays.size();
+        const size_t n = p.req.n_rays;
+        if (!rc_rays && p.req.out_hits && n)
+          std::memcpy(p.req.out_hits, p.pin != kNoPin ? s->pin_hits + p.pin : h_hits + at, n * sizeof(vx_hit));
+        at += n;
       }
-      if (!rc && hipMemcpyAsync(d_rays, h_rays, nr * sizeof(vx_ray), hipMemcpyHostToDevice, s->qstream) != hipSuccess)
-        rc = vx_fail(VX_E_DEVICE, "server: ray upload");
-      // One fused launch per run of consecutive requests that share the scene
-      // fields of their camera (normally a single launch): a request is
-      // traced against the scene its own camera names.
-      for (size_t i = 0, r0 = 0; i < rays.size() && !rc; r0 += rays[i].rays.size(), ++i) {
-        size_t j = i, n_run = rays[i].rays.size();
-        while (j + 1 < rays.size() && same_scene(rays[j + 1].req.camera, rays[i].req.camera)) n_run += rays[++j].rays.size();
-        if (n_run) rc = vx_query_rays(s->ctx, &rays[i].req.camera, d_rays + r0, (uint32_t)n_run, d_hits + r0, s->qstream);
-        for (; i < j; ++i) r0 += rays[i].rays.size();
+      const double tr = now_ms();
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (rc_rays) s->last_error = rc_rays;
+        for (Pending& p : rays) complete(s, p, rc_rays, t0, tr, s->lat_rays);
       }
-      // Hit downloads queue behind the query on its stream (only the ranges of
-      // requests that asked for their hits), overlapping the view passes.
-      size_t off = 0;
-      for (Pending& p : rays) {
-        if (!rc && p.req.out_hits && !p.rays.empty() &&
-            hipMemcpyAsync(h_hits + off, d_hits + off, p.rays.size() * sizeof(vx_hit), hipMemcpyDeviceToHost,
-                           s->qstream) != hipSuccess)
-          rc = vx_fail(VX_E_DEVICE, "server: hit download");
-        off += p.rays.size();
-      }
+      s->cv.notify_all();
+      if (!rc) rc = rc_rays;
     }
     if (!rc && rendered) rc = vx_sync(s->ctx);
-    if (!rc && nr && hipStreamSynchronize(s->qstream) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "server: query stream");
-    // Outputs and results.
     for (size_t i = 0; i < views.size() && !rc; ++i)
       if (views[i].req.out_rgba)
         rc = vx_read_output(vs[i], VX_OUT_OUTPUT, views[i].req.out_rgba,
                             (size_t)views[i].req.width * views[i].req.height * 16);
-    size_t at = 0;
-    for (Pending& p : rays) {
-      if (!rc && p.req.out_hits) std::memcpy(p.req.out_hits, h_hits + at, p.rays.size() * sizeof(vx_hit));
-      at += p.rays.size();
-    }
     const double t1 = now_ms();
     {
       std::lock_guard<std::mutex> lk(s->mu);
       if (rc) s->last_error = rc;
       ++s->st.launches;
       s->st.busy_ms += t1 - t0;
-      if (!rc && s->inflight_rays && t1 > t0) {  // throughput estimate for the latency target
-        const double r = (double)s->inflight_rays / (t1 - t0);
-        s->rate = s->rate > 0.0 ? 0.8 * s->rate + 0.2 * r : r;
-      }
-      s->inflight_rays = 0;
-      auto finish = [&](Pending& p, std::vector<double>& lat) {
-        vx_result r{};
-        r.tag = p.req.tag;
-        r.kind = p.req.kind;
-        r.status = rc;
-        r.submit_ms = p.submit_ms - s->t_start;
-        r.start_ms = t0 - s->t_start;
-        r.done_ms = t1 - s->t_start;
-        s->outbox.push_back(r);
-        lat.push_back(t1 - p.submit_ms);
-        s->mem_in_use -= p.reserved;
-        ++s->st.completed;
-      };
+      if (!rc && !created_view && !s->inflight.empty() && t1 > t0) s->cost.add(s->inflight, t1 - t0);
+      s->inflight = Work{};
+      s->last_launch_end_ms = t1;
       for (Pending& p : views) {
         bool more = false;
         for (auto& q : s->inbox) more |= q.req.kind == VX_REQ_VIEW && q.req.stream_id == p.req.stream_id;
         s->streams[p.req.stream_id].busy = more;
-        s->st.rays += (uint64_t)p.req.width * p.req.height;
-        finish(p, s->lat_view);
+        complete(s, p, rc, t0, t1, s->lat_view);
       }
-      for (Pending& p : rays) {
-        s->st.rays += p.rays.size();
-        finish(p, s->lat_rays);
+      if (s->calib.size() >= kCalibMin) {
+        std::vector<double> r = s->calib;
+        const size_t k = (size_t)(0.99 * (double)(r.size() - 1));
+        std::nth_element(r.begin(), r.begin() + k, r.end());
+        s->margin = std::min(std::max(r[k], 1.0), kMarginMax);
       }
     }
     s->cv.notify_all();
@@ -740,6 +924,14 @@ int vx_server_create(vx_ctx* ctx, const vx_host_world_view* world, const vx_serv
     delete s;
     return vx_fail(VX_E_DEVICE, "vx_server_create: query stream");
   }
+  // The ray pool: the budget's worth of rays, between 64 Ki and 8 Mi.
+  const uint64_t pool = std::min<uint64_t>(std::max<uint64_t>(s->cfg.memory_budget / kRayBytes, 1ull << 16), 1ull << 23);
+  if (hipHostMalloc((void**)&s->pin_rays, pool * sizeof(vx_ray), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->pin_hits, pool * sizeof(vx_hit), hipHostMallocDefault) != hipSuccess) {
+    vx_server_destroy(s);
+    return vx_fail(VX_E_NOMEM, "vx_server_create: pinned ray pool (%llu rays)", (unsigned long long)pool);
+  }
+  s->pin.init(pool, {});
   s->device_build = (s->cfg.flags & VX_SERVER_DEVICE_BUILD) != 0;
   if (s->device_build) {
     const size_t slots = s->meta.size();
@@ -780,6 +972,8 @@ void vx_server_destroy(vx_server* s) {
   for (void* p : {(void*)s->dmat, (void*)s->dground, (void*)s->d_off, (void*)s->d_val})
     if (p) hipFree(p);
   if (s->qstream) hipStreamDestroy(s->qstream);
+  if (s->pin_rays) hipHostFree(s->pin_rays);
+  if (s->pin_hits) hipHostFree(s->pin_hits);
   vx_internal_child(s->ctx, -1);
   delete s;
 }
@@ -793,7 +987,17 @@ int vx_server_submit(vx_server* s, const vx_request* req) {
       return vx_fail(VX_E_INVALID, "vx_server_submit: bad view size");
   } else if (req->kind == VX_REQ_RAYS) {
     if (!req->rays && req->n_rays) return vx_fail(VX_E_INVALID, "vx_server_submit: rays missing");
-    p.rays.assign(req->rays, req->rays + req->n_rays);
+    if (req->n_rays) {
+      bool pinned = false;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        pinned = s->pin.alloc(req->n_rays, p.pin);
+      }
+      if (pinned)
+        std::memcpy(s->pin_rays + p.pin, req->rays, (size_t)req->n_rays * sizeof(vx_ray));
+      else
+        p.rays.assign(req->rays, req->rays + req->n_rays);
+    }
   } else {
     return vx_fail(VX_E_INVALID, "vx_server_submit: bad request kind %u", req->kind);
   }
@@ -801,6 +1005,11 @@ int vx_server_submit(vx_server* s, const vx_request* req) {
   {
     std::lock_guard<std::mutex> lk(s->mu);
     ++s->st.submitted;
+    auto reject = [&] {
+      ++s->st.rejected;
+      if (req->kind == VX_REQ_VIEW && !s->streams[req->stream_id].w) s->streams.erase(req->stream_id);
+      if (p.pin != kNoPin) s->pin.release(p.pin, req->n_rays);
+    };
     uint64_t need = 0;
     if (req->kind == VX_REQ_VIEW) {
       Stream& stp = s->streams[req->stream_id];
@@ -808,12 +1017,12 @@ int vx_server_submit(vx_server* s, const vx_request* req) {
         uint32_t q = 0;
         for (auto& p2 : s->inbox) q += p2.req.kind == VX_REQ_VIEW && p2.req.stream_id == req->stream_id;
         if (q + 1u > s->cfg.max_queued_frames) {  // counts the frame in flight too
-          ++s->st.rejected;
+          reject();
           return vx_fail(VX_E_BUSY, "vx_server_submit: stream %u already has %u frame(s) pending", req->stream_id, q + 1u);
         }
       }
       if (stp.w && (stp.w != req->width || stp.h != req->height)) {
-        ++s->st.rejected;
+        reject();
         return vx_fail(VX_E_INVALID, "vx_server_submit: stream %u is %ux%u", req->stream_id, stp.w, stp.h);
       }
       if (!stp.w) need = (uint64_t)req->width * req->height * kViewBytesPerPx;  // a new stream's resident view
@@ -821,24 +1030,36 @@ int vx_server_submit(vx_server* s, const vx_request* req) {
       need = (uint64_t)req->n_rays * kRayBytes;  // in-flight ray buffers
     }
     if (s->mem_in_use + need > s->cfg.memory_budget) {
-      ++s->st.rejected;
-      if (req->kind == VX_REQ_VIEW && !s->streams[req->stream_id].w) s->streams.erase(req->stream_id);
+      reject();
       return vx_fail(VX_E_BUSY, "vx_server_submit: %llu bytes over the %llu-byte budget",
                      (unsigned long long)(s->mem_in_use + need), (unsigned long long)s->cfg.memory_budget);
     }
-    const uint64_t n = request_rays(p);
-    if (s->cfg.latency_target_ms > 0.0f && s->rate > 0.0) {
-      // estimated finish: the rest of the launch in flight, then the queue, then this request
-      const double left = s->inflight_rays ? std::max(0.0, (double)s->inflight_rays / s->rate - (p.submit_ms - s->inflight_t0)) : 0.0;
-      const double est = left + (double)(s->queued_rays + n) / s->rate;
+    const Work w = request_work(p);
+    // A server that has been idle for kProbeAfterMs admits the request
+    // whatever the estimate says: it runs alone, and its launch refreshes the
+    // cost model (which otherwise could stay pessimistic for good once every
+    // request is rejected).
+    const bool probe = s->inbox.empty() && s->inflight.empty() && p.submit_ms - s->last_launch_end_ms > kProbeAfterMs;
+    if (s->cfg.latency_target_ms > 0.0f && s->cost.ready() && !probe) {
+      // Estimated finish: the rest of the launch in flight, then the launches
+      // the queue and this request need (each at most the launch cap), each
+      // with its fixed cost.
+      const double left =
+          s->inflight.empty() ? 0.0 : std::max(0.0, s->cost.ms(s->inflight) - (p.submit_ms - s->inflight_t0));
+      Work ahead = s->queued;
+      ahead += w;
+      const double work_ms = s->cost.rays_ms(ahead);
+      const double per_launch = std::max(1e-3, kLaunchShare * s->cfg.latency_target_ms - s->cost.fixed());
+      const double launches = std::max(1.0, std::ceil(work_ms / per_launch));
+      p.est_ms = left + launches * s->cost.fixed() + work_ms;
+      const double est = s->margin * p.est_ms;
       if (est > s->cfg.latency_target_ms) {
-        ++s->st.rejected;
-        if (req->kind == VX_REQ_VIEW && !s->streams[req->stream_id].w) s->streams.erase(req->stream_id);
+        reject();
         return vx_fail(VX_E_BUSY, "vx_server_submit: estimated %.1f ms over the %.1f ms latency target", est,
                        (double)s->cfg.latency_target_ms);
       }
     }
-    s->queued_rays += n;
+    s->queued += w;
     s->mem_in_use += need;
     if (req->kind == VX_REQ_VIEW) {
       Stream& stp = s->streams[req->stream_id];
