@@ -28,3 +28,15 @@ inline int32_t min(int32_t a, int32_t b) { return a < b ? a : b; }
 inline int32_t max(int32_t a, int32_t b) { return a > b ? a : b; }
 // LDS atomics: each host task owns its workgroup's LDS, so a plain add is exact.
 inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { uint32_t o = *p; *p = o + v; return o; }
+// Wavefront primitives: each host lane runs as a wavefront of one lane (the
+// lane id is the one the host loop set), so a cross-lane read returns the
+// lane's own value and a ballot holds its own bit.
+extern thread_local uint32_t vxd_cpu_lane;
+#define __builtin_amdgcn_read_exec() (1ull << vxd_cpu_lane)
+inline uint32_t __lane_id() { return vxd_cpu_lane; }
+inline unsigned long long __ballot(int p) { return p ? 1ull << vxd_cpu_lane : 0ull; }
+inline int __shfl(int v, int, int) { return v; }
+inline float __shfl(float v, int, int) { return v; }
+#define __builtin_amdgcn_ds_permute(addr, v) (v)
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline void __threadfence_block() {}

@@ -26,7 +26,19 @@ namespace counted {
 #include "vxd_passes.inc"
 #undef VXD_COUNT
 }  // namespace counted
+// The product form with 2 deferred grass tests per lane and attempt, so that
+// chunks take repeated attempts (mode 3).
+namespace plain_k2 {
+#define VXD_COUNT 0
+#define VXD_DEFER_K 2
+#include "vxd_passes.inc"
+#undef VXD_DEFER_K
+#undef VXD_COUNT
+}  // namespace plain_k2
 }  // namespace vxd
+
+// The lane the host loops run (each lane is a wavefront of one).
+thread_local uint32_t vxd_cpu_lane = 0;
 
 using namespace vxd;
 
@@ -110,11 +122,12 @@ void fill_taper(float* taper) {
 
 extern "C" {
 
-// mode 0: the one-kernel pass (k_primary<false>), which adds the profile
-// counters into prof[5] when cam->profile_flags & 1 (as k_primary_batch does
-// through each tile's LDS counters); 2: the load-counting kernel
-// (k_primary_count_reference), which also adds its counts into counts[9] and
-// its part byte totals into counts[9..17).
+// mode 0: the product pass (k_primary_batch), which adds the profile counters
+// into prof[5] when cam->profile_flags & 1 (as k_primary_batch does through
+// each tile's LDS counters); 3: the same with 2 deferred grass tests per lane
+// and attempt; 2: the load-counting kernel (k_primary_count_reference), which
+// also adds its counts into counts[9] and its part byte totals into
+// counts[9..17).
 int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, uint32_t h, const float* primary_hist_in,
                     const float* shadow_in, float* color, float* depth, float* local, float* primary_hist_out,
                     float* shadow_out, vx_hit* hits, int mode, int threads, uint64_t* counts, uint64_t* prof) {
@@ -137,35 +150,46 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
   std::vector<Counts> cnt((size_t)threads, Counts{});
   std::vector<uint64_t> prof_t((size_t)threads * 5, 0u);
 
-  if (mode == 0 || mode == 2) {
+  // The product pass for one tile in namespace NS (its wavefront gather, then
+  // each lane's pixel with the lane's own deferral column).
+#define VXD_CPU_PRODUCT_TILE(NS)                                                                                     \
+  do {                                                                                                               \
+    NS::Ctx C;                                                                                                       \
+    C.S = S;                                                                                                         \
+    C.cam = cam;                                                                                                     \
+    C.grass_taper = L.grass_taper;                                                                                   \
+    for (uint32_t& v : L.prof) v = 0u;                                                                               \
+    C.prof = (cam->profile_flags & 1u) ? L.prof : nullptr;                                                           \
+    /* k_primary_batch's wavefront gather: each phase for every lane, the barrier between phases being the end of  \
+       each lane loop */                                                                                             \
+    for (uint32_t lane = 0; lane < 64u; ++lane) NS::gather_walk(C, L, wgx, wgy, lane);                               \
+    if (NS::gather_overflow(L)) {                                                                                    \
+      NS::tile_build(C, L, wgx, wgy);                                                                                \
+    } else {                                                                                                         \
+      for (uint32_t lane = 0; lane < 64u; ++lane) NS::gather_load(C, L, lane);                                       \
+      for (uint32_t lane = 0; lane < 64u; ++lane) NS::gather_first(L, lane);                                         \
+      for (uint32_t lane = 0; lane < 64u; ++lane) NS::gather_place(L, lane);                                         \
+      NS::gather_finish(L);                                                                                          \
+    }                                                                                                                \
+    std::vector<uint4> defer((size_t)kDeferEntries * 64u);                                                           \
+    for (uint32_t lane = 0; lane < 64u; ++lane) {                                                                    \
+      const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);                                       \
+      vxd_cpu_lane = lane;                                                                                           \
+      C.grass_blades = L.grass_blades + lane;                                                                        \
+      C.defer = defer.data() + lane;                                                                                 \
+      if (gx < w && gy < h) NS::primary_pixel(C, L, fb, gx, gy, true, NS::sky_bg(F3(0.0f, 1.0f, 0.0f)));             \
+    }                                                                                                                \
+    for (int i = 0; i < 5; ++i) prof_t[(size_t)t * 5 + i] += L.prof[i];                                              \
+  } while (0)
+  if (mode == 0 || mode == 2 || mode == 3) {
     parallel_for(nt, threads, [&](uint32_t tile, int t) {
       TileLDS L;
       fill_taper(L.grass_taper);
       const uint32_t wgx = tile % tx, wgy = tile / tx;
       if (mode == 0) {
-        plain::Ctx C;
-        C.S = S;
-        C.cam = cam;
-        C.grass_taper = L.grass_taper;
-        for (uint32_t& v : L.prof) v = 0u;
-        C.prof = (cam->profile_flags & 1u) ? L.prof : nullptr;
-        // k_primary_batch's wavefront gather: each phase for every lane, the
-        // barrier between phases being the end of each lane loop.
-        for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_walk(C, L, wgx, wgy, lane);
-        if (plain::gather_overflow(L)) {
-          plain::tile_build(C, L, wgx, wgy);
-        } else {
-          for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_load(C, L, lane);
-          for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_first(L, lane);
-          for (uint32_t lane = 0; lane < 64u; ++lane) plain::gather_place(L, lane);
-          plain::gather_finish(L);
-        }
-        for (uint32_t lane = 0; lane < 64u; ++lane) {
-          const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
-          C.grass_blades = L.grass_blades + lane;
-          if (gx < w && gy < h) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
-        }
-        for (int i = 0; i < 5; ++i) prof_t[(size_t)t * 5 + i] += L.prof[i];
+        VXD_CPU_PRODUCT_TILE(plain);
+      } else if (mode == 3) {
+        VXD_CPU_PRODUCT_TILE(plain_k2);
       } else {
         counted::Ctx C;
         C.S = S;
@@ -176,13 +200,14 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
         for (uint32_t lane = 0; lane < 64u; ++lane) {
           const uint32_t gx = wgx * 8u + (lane & 7u), gy = wgy * 8u + (lane >> 3);
           C.grass_blades = L.grass_blades + lane;
-          if (gx < w && gy < h) counted::primary_pixel(C, L, fb, gx, gy, counted::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+          if (gx < w && gy < h) counted::primary_pixel(C, L, fb, gx, gy, true, counted::sky_bg(F3(0.0f, 1.0f, 0.0f)));
         }
       }
     });
   } else {
     return -1;
   }
+#undef VXD_CPU_PRODUCT_TILE
   if (prof)
     for (int t = 0; t < threads; ++t)
       for (int i = 0; i < 5; ++i) prof[i] += prof_t[(size_t)t * 5 + i];

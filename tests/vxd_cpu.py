@@ -18,7 +18,7 @@ OUT = HERE / "cpu_kernels" / "_build" / "libvxd_cpu.so"
 CSRC = REPO / "voxel-engine-v0_amd" / "csrc"
 DEPS = [SRC, CSRC / "vxd_passes.inc", CSRC / "vxd_math.h", CSRC / "vxd_consts.h", CSRC / "vxd_scene.h",
         REPO / "include" / "voxhip.h", HERE / "cpu_kernels" / "shim" / "hip" / "hip_runtime.h"]
-MODES = {"one": 0, "counted": 2}
+MODES = {"one": 0, "counted": 2, "one_k2": 3}
 
 
 def build() -> Path:

@@ -12,6 +12,8 @@
 //     workgroups over 16x16 pixel blocks.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "vx_launch.h"
 #include "vx_terrain.h"
 #include "vxd_consts.h"
@@ -106,7 +108,7 @@ __device__ __forceinline__ void flush_counts(const Counts& c, unsigned long long
     if (threadIdx.x == 0) NS::tile_build(C, L, wgx, wgy);                                                              \
     const uint32_t tile_b = threadIdx.x == 0 ? count_bytes(cnt) : 0u; /* lane 0: the tile's gather so far */          \
     __syncthreads();                                                                                                   \
-    if (gx < fb.rw && gy < fb.rh) NS::primary_pixel(C, L, fb, gx, gy, NS::sky_bg(F3(0.0f, 1.0f, 0.0f)));               \
+    if (gx < fb.rw && gy < fb.rh) NS::primary_pixel(C, L, fb, gx, gy, true, NS::sky_bg(F3(0.0f, 1.0f, 0.0f)));         \
     const float share = (float)__shfl(tile_b, 0, 64) * (1.0f / 64.0f);                                                 \
     if (ray_bytes && gx < fb.rw && gy < fb.rh)                                                                         \
       ray_bytes[(size_t)gy * fb.rw + gx] = (float)(count_bytes(cnt) - tile_b) + share;                                 \
@@ -222,6 +224,10 @@ __global__ __launch_bounds__(64) void k_batch_end(Batch B) {
 template <bool PROF>
 __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scene S, Batch B) {
   __shared__ TileLDS L;
+  // One deferral region per workgroup (the launchers never start more
+  // workgroups than there are regions).
+  if (blockIdx.x >= S.n_defer_regions) return;
+  uint4* const defer = S.defer + (size_t)blockIdx.x * (kDeferEntries * 64u) + threadIdx.x;
   if (threadIdx.x < 12u) L.grass_taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
   if (threadIdx.x < 5u) L.prof[threadIdx.x] = 0u;
   uint32_t band = blockIdx.x % kBatchBands, steps = 0u, t0 = 0u, t1 = 0u;
@@ -237,6 +243,7 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
       C.cam = V.cam;
       C.grass_taper = L.grass_taper;
       C.grass_blades = L.grass_blades + threadIdx.x;
+      C.defer = defer;
       // Profiling (ray_main.wgsl:64-74) is a property of the view's camera, so
       // uniform over the tile: the counters go to LDS and are flushed per tile.
       const bool prof = PROF && (V.cam->profile_flags & 1u) != 0u;
@@ -260,8 +267,10 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
         if (threadIdx.x == 0) plain::gather_finish(L);
       }
       __syncthreads();
+      // Every lane runs the pixel pass (those outside the image only help with
+      // the deferred grass tests, which need the whole wavefront).
       const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
-      if (gx < fb.rw && gy < fb.rh) plain::primary_pixel(C, L, fb, gx, gy, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
+      plain::primary_pixel(C, L, fb, gx, gy, gx < fb.rw && gy < fb.rh, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
       if (prof) {
         __syncthreads();  // every lane's LDS increments are in
         if (threadIdx.x < 5u) {
@@ -429,8 +438,9 @@ hipError_t vx_launch_batch(const vxd::Scene& S, const vxd::Scene& S_godray, cons
                            uint32_t persistent, bool profile, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, B);
   if (mask & VX_PASS_PRIMARY) {
-    if (profile) hipLaunchKernelGGL(k_primary_batch<true>, dim3(persistent), dim3(64), 0, st, S, B);
-    else hipLaunchKernelGGL(k_primary_batch<false>, dim3(persistent), dim3(64), 0, st, S, B);
+    const uint32_t groups = std::min(persistent, S.n_defer_regions);
+    if (profile) hipLaunchKernelGGL(k_primary_batch<true>, dim3(groups), dim3(64), 0, st, S, B);
+    else hipLaunchKernelGGL(k_primary_batch<false>, dim3(groups), dim3(64), 0, st, S, B);
   }
   if (mask & VX_PASS_LOCAL_TAA) hipLaunchKernelGGL(k_local_taa_batch, dim3(B.rblks), dim3(256), 0, st, B);
   if (mask & VX_PASS_GODRAY) hipLaunchKernelGGL(k_godray_batch, dim3(B.tiles), dim3(64), 0, st, S_godray, B);
@@ -447,8 +457,9 @@ hipError_t vx_launch_ws_begin(const vxd::Batch& B, hipStream_t st) {
 }
 hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, bool profile,
                                 hipStream_t st) {
-  if (profile) hipLaunchKernelGGL(k_primary_batch<true>, dim3(persistent), dim3(64), 0, st, S, B);
-  else hipLaunchKernelGGL(k_primary_batch<false>, dim3(persistent), dim3(64), 0, st, S, B);
+  const uint32_t groups = std::min(persistent, S.n_defer_regions);
+  if (profile) hipLaunchKernelGGL(k_primary_batch<true>, dim3(groups), dim3(64), 0, st, S, B);
+  else hipLaunchKernelGGL(k_primary_batch<false>, dim3(groups), dim3(64), 0, st, S, B);
   return hipGetLastError();
 }
 

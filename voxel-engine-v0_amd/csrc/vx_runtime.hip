@@ -90,6 +90,10 @@ struct vx_ctx {
   // from the node arena and the chunk table after they change).
   uint32_t* chunk_light = nullptr;
   std::atomic<bool> lights_dirty{true};
+  // The primary kernel's deferred grass tests: kDeferRegionsPerCU regions of
+  // kDeferEntries x 64 entries per CU (16 KiB each; 64 MiB on 256 CUs).
+  uint4* defer = nullptr;
+  uint32_t defer_regions = 0;
   // Ray queries: the scene camera copy, the ray cursor, configuration, timing.
   vx_camera* qcam_dev = nullptr;
   uint32_t* qcursor = nullptr;
@@ -199,6 +203,8 @@ vxd::Scene make_scene(const vx_ctx* c, bool godray) {
   S.n_colinfo = (uint64_t)c->chunk_cap * 2048u;
   S.n_clip = (uint64_t)c->clip_levels * c->clip_res * c->clip_res;
   S.chunk_light = c->chunk_light;
+  S.defer = c->defer;
+  S.n_defer_regions = c->defer_regions;
   if (godray && !(c->flags & VX_CTX_GODRAY_MACRO_FIX)) {
     // The reference's godray pass sees the rope buffer where it expects macro_occ.
     S.macro = reinterpret_cast<const uint32_t*>(c->ropes);
@@ -315,6 +321,13 @@ int vx_create(const vx_caps* caps, vx_ctx** out) {
     vx_destroy(ctx);
     return rc;
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0) cus = 256;
+  ctx->defer_regions = (uint32_t)cus * vxd::kDeferRegionsPerCU;
+  if ((rc = dalloc(&ctx->defer, (size_t)ctx->defer_regions * vxd::kDeferEntries * 64u))) {
+    vx_destroy(ctx);
+    return rc;
+  }
   vx_clipmap empty;
   std::memset(&empty, 0, sizeof(empty));
   empty.res = ctx->clip_res;
@@ -352,6 +365,7 @@ int vx_destroy(vx_ctx* ctx) {
   dfree(ctx->clip_dev);
   dfree(ctx->clip_rects);
   dfree(ctx->chunk_light);
+  dfree(ctx->defer);
   dfree(ctx->qcam_dev);
   dfree(ctx->qcursor);
   for (auto& e : ctx->qev)

@@ -217,4 +217,17 @@ VXD float vx_cos(float x) {
   return (x == x && fabsf(x) != __builtin_inff()) ? v : __builtin_nanf("");
 }
 
+// ---- wavefront primitives ----
+// The primary kernel's workgroup is one 64-lane wavefront.  Cross-lane reads
+// take their source from active lanes only.
+VXD uint64_t wave_exec() { return __builtin_amdgcn_read_exec(); }
+VXD uint32_t wave_lane() { return __lane_id(); }
+VXD uint64_t wave_ballot(bool p) { return __ballot(p ? 1 : 0); }
+VXD bool wave_any(bool p) { return wave_ballot(p) != 0ull; }
+VXD uint32_t wave_read(uint32_t v, uint32_t src) { return (uint32_t)__shfl((int)v, (int)src, 64); }
+VXD float wave_read(float v, uint32_t src) { return __shfl(v, (int)src, 64); }
+// Lane `dst` receives v (each active lane sends to one lane).
+VXD uint32_t wave_send(uint32_t dst, uint32_t v) { return (uint32_t)__builtin_amdgcn_ds_permute((int)(dst << 2), (int)v); }
+VXD uint32_t popc64(uint64_t x) { return (uint32_t)__popcll(x); }
+
 }  // namespace vxd

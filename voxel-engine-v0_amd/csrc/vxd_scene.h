@@ -22,7 +22,19 @@ struct Scene {
   // from the node arena and the chunk table, never uploaded).  Null: unknown,
   // so every local-light gather runs in full.
   const uint32_t* __restrict__ chunk_light;
+  // The primary kernel's deferred grass tests: one region of kDeferEntries x
+  // 64 entries per workgroup (entry e of lane l at [e * 64 + l]), for at most
+  // n_defer_regions workgroups.
+  uint4* __restrict__ defer;
+  uint32_t n_defer_regions;
 };
+
+// Grass layer tests the primary traversal records per lane and per attempt
+// at a chunk before the wavefront runs them together.
+constexpr uint32_t kDeferEntries = 16;
+// Workgroups per CU the primary kernel's deferral regions are sized for: its
+// register budget (128 VGPRs, 4 waves per SIMD) and LDS allow at most 16.
+constexpr uint32_t kDeferRegionsPerCU = 16;
 
 // Parts of the primary pass that loads are charged to (VX_COUNT_PARTS):
 // the tile's candidate gather, the voxel traversal, grass (column probes and
