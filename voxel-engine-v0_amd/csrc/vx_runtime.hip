@@ -91,7 +91,7 @@ struct vx_ctx {
   uint32_t* chunk_light = nullptr;
   std::atomic<bool> lights_dirty{true};
   // The primary kernel's deferred grass tests: kDeferRegionsPerCU regions of
-  // kDeferEntries x 64 entries per CU (16 KiB each; 64 MiB on 256 CUs).
+  // kDeferEntries x 64 entries per CU (32 KiB each; 128 MiB on 256 CUs).
   uint4* defer = nullptr;
   uint32_t defer_regions = 0;
   // Ray queries: the scene camera copy, the ray cursor, configuration, timing.

@@ -31,7 +31,7 @@ struct Scene {
 
 // Grass layer tests the primary traversal records per lane and per attempt
 // at a chunk before the wavefront runs them together.
-constexpr uint32_t kDeferEntries = 16;
+constexpr uint32_t kDeferEntries = 32;
 // Workgroups per CU the primary kernel's deferral regions are sized for: its
 // register budget (128 VGPRs, 4 waves per SIMD) and LDS allow at most 16.
 constexpr uint32_t kDeferRegionsPerCU = 16;
