@@ -323,25 +323,37 @@ def main():
             passes[name] = {"bytes_per_launch": int(bts), "ms": round(float(ms), 4), "achieved_gbs": round(gbs, 1),
                             "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
-    lat1080 = None if a.no_1080p else latency_1080p(vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup,
-                                                    not a.static_clipmap)
-    clipmap = None if a.static_clipmap else clipmap_costs(vx, sc, ctx)
-    batched = None if a.batch_views <= 0 else batching(vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps)
-    queries = None if a.query_batches <= 0 else ray_queries(vx, scenes, a.query_batches)
-    mix = None if a.no_mixed else mixed(vx, scenes)
-    serving = None if a.serve_seconds <= 0 else serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms)
+    def leg(name, fn):
+        # One progress line per workload, on stderr (stdout carries only the JSON line).
+        t = time.time()
+        r = fn()
+        print(f"bench.py: {name} done in {time.time() - t:.1f} s", file=sys.stderr, flush=True)
+        return r
+
+    print(f"bench.py: {W}x{H} flight frames done ({total_s:.1f} s timed)", file=sys.stderr, flush=True)
+    lat1080 = None if a.no_1080p else leg("1080p latency", lambda: latency_1080p(
+        vx, scenes, sc, hw, ctx, max(a.steps, 20), a.warmup, not a.static_clipmap))
+    clipmap = None if a.static_clipmap else leg("clipmap update", lambda: clipmap_costs(vx, sc, ctx))
+    batched = None if a.batch_views <= 0 else leg("batching", lambda: batching(
+        vx, scenes, sc, hw, ctx, a.batch_views, a.batch_steps))
+    queries = None if a.query_batches <= 0 else leg("ray queries", lambda: ray_queries(vx, scenes, a.query_batches))
+    mix = None if a.no_mixed else leg("mixed", lambda: mixed(vx, scenes))
+    serving = None if a.serve_seconds <= 0 else leg("serving", lambda: serve(
+        vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms))
     serving_untargeted = serving_device = None
     if a.serve_seconds > 0 and a.serve_target_ms > 0:
-        serving_untargeted = serve(vx, scenes, a.serve_seconds, target_ms=0.0)
+        serving_untargeted = leg("serving, no target", lambda: serve(vx, scenes, a.serve_seconds, target_ms=0.0))
     if a.serve_seconds > 0:
-        serving_device = serve(vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms, device_build=True)
-    serving1080 = None if a.serve_seconds <= 0 else serve_1080p(vx, scenes, 4.0)
-    builds = None if a.no_chunk_builds else chunk_builds(vx, scenes)
+        serving_device = leg("serving, device builds", lambda: serve(
+            vx, scenes, a.serve_seconds, target_ms=a.serve_target_ms, device_build=True))
+    serving1080 = None if a.serve_seconds <= 0 else leg("serving 1080p", lambda: serve_1080p(vx, scenes, 4.0))
+    builds = None if a.no_chunk_builds else leg("chunk builds", lambda: chunk_builds(vx, scenes))
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(sc, hw, clip, heights, cams[a.warmup], W, H, a.cpu_rows, frame=a.warmup)
-    c0 = None if a.no_cpu_baseline else config0(vx, scenes)
+        cpu = leg("cpu baseline", lambda: cpu_baseline(sc, hw, clip, heights, cams[a.warmup], W, H, a.cpu_rows,
+                                                       frame=a.warmup))
+    c0 = None if a.no_cpu_baseline else leg("config 0", lambda: config0(vx, scenes))
 
     out = {
         "metric": "primary rays/s at 4K + p50/p99 per-view latency, served batched vs plain (1 GPU)",

@@ -5,6 +5,20 @@
 //
 // TEST INFRASTRUCTURE ONLY.  This is the checker of the HIP passes.  It is
 // never linked into, loaded by, or called from the product library.
+//
+// How each part is pinned independently of this file:
+//   traversal, queries        dense brute-force voxel walks (test_traversal.py,
+//                             test_query_oracle.py)
+//   composite, both TAAs, the godray history blend, the cloud-free sky
+//                             float64 statements (test_oracle_composition.py)
+//   sky cloud march, godray march with fog on (empty world and a slab roof),
+//   dirt/stone/wood voxel shading, grass heightfield shading
+//                             float64 statements (test_oracle_shading.py)
+//   leaf helpers              closed forms (test_oracle_math.py)
+// Parity unpinned: the wind field away from the ground (hash1 of a sin, so its
+// value follows the sin implementation), the voxel-grass shading extras
+// (wrapped diffuse, back-scatter, grass self-shadow), the leaf dapple and the
+// displaced leaf cubes; the exact transcendental values.
 #include <algorithm>
 #include <atomic>
 #include <thread>
