@@ -525,8 +525,10 @@ def mixed(vx, scenes, n_views=64, n_batches=256, steps=2):
     the 32x4x32-chunk seed-42 world with views along seeded flights.  Plain: one
     launch sequence per view (vx_render) and one query launch per ray batch,
     all on the context stream.  Batched: the views as one fused batch
-    (vx_batch_render, autotuned) on the context stream while all ray batches
-    run as one fused query launch (autotuned) on a second stream."""
+    (vx_batch_render, autotuned), with all ray batches as one set of rays
+    traced inside the views' persistent primary launch (vx_batch_set_rays),
+    so ray blocks fill the time the last view tiles take; the batch and the
+    query configuration are autotuned."""
     sc = scenes.c4(streams=n_views, frames=200)
     res, _ = scenes.batch_views(n_views, len(sc.poses))
     sizes_q, rays = scenes.query_batches(n_batches, sc.grid_origin, sc.grid_dims)
@@ -583,23 +585,24 @@ def mixed(vx, scenes, n_views=64, n_batches=256, steps=2):
     batch = vx.Batch(ctx, views)
     for v, c in zip(views, cams(0)):
         v.write_camera(c)
-    tuned_views = batch.tune(vx.PASS_FRAME, iters=1).as_dict()
+    # The ray batches ride in the views' persistent primary launch
+    # (vx_batch_set_rays): the query kernel's configuration is tuned alone,
+    # then the fused launch with the rays attached.
     tuned_rays = vx.query_tune(ctx, qcam, d_rays, d_hits, n_rays).as_dict()
-    qs = vx.HipStream()
+    batch.set_rays(qcam, d_rays, d_hits, n_rays)
+    tuned_views = batch.tune(vx.PASS_FRAME, iters=2).as_dict()
 
     def batched_step(cs):
         for v, c in zip(views, cs):
             v.write_camera(c)
         batch.render(vx.PASS_FRAME)
-        vx.query_rays_device(ctx, qcam, d_rays, d_hits, n_rays, stream=qs.handle.value)
         ctx.sync()
-        qs.sync()
 
     t_batch = run(batched_step)
+    batch.set_rays(None, None, None, 0)
     batch.close()
     for v in views:
         v.close()
-    qs.close()
     d_rays.close()
     d_hits.close()
     ctx.close()

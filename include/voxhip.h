@@ -365,6 +365,19 @@ int  vx_batch_get_config(const vx_batch* batch, vx_batch_config* cfg);
  * waves_per_simd in {4, 6, 8} x grab in {1, 2}, `iters` runs each,
  * keeps the fastest configuration and returns it in *chosen (may be NULL). */
 int  vx_batch_tune(vx_batch* batch, uint32_t pass_mask, uint32_t iters, vx_batch_config* chosen);
+/* No counterpart in the reference: ray batches fused into a batch's launch.
+ * Each later vx_batch_render whose pass_mask holds VX_PASS_PRIMARY also
+ * traces these n rays inside its persistent primary launch: workgroups take
+ * 64-ray blocks (with the lane refill of vx_query_rays and the context's
+ * query configuration) once every view tile has been taken, so the rays fill
+ * the time the last tiles take.  hits_dev receives exactly what vx_query_rays
+ * would write for the same scene_cam and rays.  The arrays stay the caller's
+ * and must stay valid until the render has completed on its stream; a render
+ * without a new call traces the same rays again.  n = 0 detaches them.  The
+ * call waits until the last render that uploaded the previous rays has
+ * started reading them. */
+int  vx_batch_set_rays(vx_batch* batch, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n,
+                       vx_hit* hits_dev);
 
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops, harnesses that forbid

@@ -72,3 +72,81 @@ def test_batch_equals_single_views(gpu):
                     which = getattr(gpu, o)
                     assert _same(a.read(which), b.read(which)), (name, frame, i, o)
         print(f"[batch] frame {frame}: all {len(SIZES)} views identical in {len(sets)} batch configurations", flush=True)
+
+
+def _random_rays(rng, n, grid_dims, tmax=51.2):
+    """Origins uniform in the world AABB, directions uniform on the sphere (the ray-query workload's shape)."""
+    ext = np.array(grid_dims, np.float64) * 6.4
+    o = rng.uniform(0.0, 1.0, (n, 3)) * ext
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3], r[:, 3], r[:, 4:7] = o, tmax, d
+    return r
+
+
+def test_batch_with_fused_rays(gpu):
+    """Ray batches fused into a batch launch (vx_batch_set_rays): the hits
+    equal vx_query_rays on the same rays byte for byte (and the oracle on a
+    sample and on the boundary rays), the views' outputs are those of the
+    same batch without rays, replacing the rays between frames takes effect,
+    and n = 0 detaches them; with a graph and with direct launches."""
+    import worlds
+    from oracle import oracle as O
+    from voxhip import scenes
+
+    sc, hw = host_world("c1")
+    clip, heights = scenes.scene_clipmap(sc, 0)
+    ctx = make_context(gpu, hw)
+    ctx.write_clipmap(clip, heights)
+    qcam = gpu.camera_build(gpu.make_pose((1.0, 1.0, 1.0), 0.0, 0.0), (64, 64), (64, 64), 0, grid_origin=sc.grid_origin,
+                            grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
+    rng = np.random.default_rng(17)
+    ray_sets = [np.concatenate([_random_rays(rng, 150_000, sc.grid_dims), worlds.boundary_rays()]),
+                _random_rays(rng, 70_001, sc.grid_dims)]
+    w = O.World.from_host(hw, clip, heights)
+    for cfg in (dict(waves_per_simd=4, grab=1, use_graph=True), dict(waves_per_simd=6, grab=2, use_graph=False)):
+        plain_views = [gpu.View(ctx, p, r) for p, r in SIZES]
+        fused_views = [gpu.View(ctx, p, r) for p, r in SIZES]
+        plain, fused = gpu.Batch(ctx, plain_views), gpu.Batch(ctx, fused_views)
+        plain.set_config(**cfg)
+        fused.set_config(**cfg)
+        for frame in range(3):
+            rays = ray_sets[frame % 2]
+            n = len(rays)
+            d_rays = gpu.DeviceArray.from_numpy(rays)
+            d_hits = gpu.DeviceArray.from_numpy(np.zeros(n, gpu.HIT_DTYPE))
+            d_ref = gpu.DeviceArray(n * gpu.HIT_DTYPE.itemsize)
+            fused.set_rays(qcam, d_rays, d_hits, n if frame < 2 else 0)
+            cams = _cams(gpu, sc, hw, frame)
+            for vs in (plain_views, fused_views):
+                for v, c in zip(vs, cams):
+                    v.write_camera(c)
+            plain.render(gpu.PASS_FRAME)
+            fused.render(gpu.PASS_FRAME)
+            gpu.query_rays_device(ctx, qcam, d_rays, d_ref, n)
+            ctx.sync()
+            for i, (a, b) in enumerate(zip(plain_views, fused_views)):
+                for o in OUTS:
+                    which = getattr(gpu, o)
+                    assert _same(a.read(which), b.read(which)), (cfg, frame, i, o)
+            ref = d_ref.download(np.empty(n, gpu.HIT_DTYPE))
+            got = d_hits.download(np.empty(n, gpu.HIT_DTYPE))
+            if frame < 2:
+                assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), (cfg, frame)
+                sample = np.sort(np.random.default_rng(frame).choice(n, 4000, replace=False))
+                if frame == 0:
+                    sample = np.union1d(sample, np.arange(150_000, n))  # the boundary rays
+                oref = O.query_rays(w, qcam, rays[sample], gpu.HIT_DTYPE)
+                assert np.array_equal(got[sample].view(np.uint8), oref.view(np.uint8)), (cfg, frame)
+                hit_frac = float(np.mean(got["kind"] == 1))
+                assert 0.2 < hit_frac < 0.98
+            else:
+                assert not got.view(np.uint8).any()   # detached: the launch traced no rays
+            for d in (d_rays, d_hits, d_ref):
+                d.close()
+            print(f"[batch] fused rays {cfg}: frame {frame}: {n} rays, views identical to the batch without rays",
+                  flush=True)
+        fused.set_rays(None, None, None, 0)
+        plain.close()
+        fused.close()

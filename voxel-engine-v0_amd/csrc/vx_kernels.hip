@@ -168,6 +168,76 @@ __global__ __launch_bounds__(256) void k_composite_taa(Scene S, const vx_camera*
 }
 
 // ---------------------------------------------------------------------------
+// Ray queries: one persistent work-stealing launch over all rays.  Each lane
+// owns one ray and advances it one chunk per loop trip; lanes whose ray has
+// finished are refilled with new rays (one wavefront-aggregated atomic on the
+// ray cursor) whenever fewer than `refill_below` lanes are still busy, so a
+// wavefront does not idle behind its longest ray.  Workgroups exit when the
+// ray cursor has passed n and none of their lanes holds a ray.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__global__ __launch_bounds__(64) void k_query_begin(uint32_t* __restrict__ cursor) {
+  if (threadIdx.x == 0) cursor[0] = 0u;
+}
+
+// The persistent lane-refill loop over rays [0, n) from `cursor` (shared by
+// the ray-query kernel and the fused batch launch).  Returns when the cursor
+// has passed n and no lane of the wavefront holds a ray.
+__device__ __forceinline__ void query_persistent(const plain::Ctx& C, const vx_ray* __restrict__ rays,
+                                                 vx_hit* __restrict__ hits, uint32_t n, uint32_t* __restrict__ cursor,
+                                                 uint32_t refill_below) {
+  plain::QueryRay q;
+  bool active = false, drained = false;
+  uint32_t id = 0u;
+  const unsigned long long below = (1ull << lane_id()) - 1ull;
+  for (;;) {
+    const uint32_t busy = (uint32_t)__popcll(__ballot(active));
+    if (busy < refill_below) {
+      const bool need = !active && !drained;
+      const unsigned long long m = __ballot(need);
+      if (m != 0ull) {
+        uint32_t base = 0u;
+        if (lane_id() == 0u) base = atomicAdd(cursor, (uint32_t)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (need) {
+          id = base + (uint32_t)__popcll(m & below);
+          if (id < n) {
+            const vx_ray r = rays[id];
+            if (plain::query_begin(C, F3(r.ox, r.oy, r.oz), F3(r.dx, r.dy, r.dz), r.tmax, q)) active = true;
+            else hits[id] = plain::query_result(plain::miss_hitgeom());
+          } else {
+            drained = true;
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0ull) {
+      if (__ballot(!drained) == 0ull) break;
+      continue;
+    }
+    if (active) {
+      plain::HitGeom h;
+      if (plain::query_step(C, q, h)) {
+        hits[id] = plain::query_result(h);
+        active = false;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_query(
+    Scene S, const vx_camera* __restrict__ cam, const vx_ray* __restrict__ rays, vx_hit* __restrict__ hits, uint32_t n,
+    uint32_t* __restrict__ cursor, uint32_t refill_below) {
+  plain::Ctx C;
+  C.S = S;
+  C.cam = cam;
+  C.grass_taper = nullptr;
+  C.geom_only = true;
+  query_persistent(C, rays, hits, n, cursor, refill_below);
+}
+
+// ---------------------------------------------------------------------------
 // Fused multi-view launches (a batch of views rendered together).
 //
 // Primary and godray run as persistent work-stealing kernels over the union
@@ -213,6 +283,7 @@ __device__ __forceinline__ bool batch_take(const Batch& B, uint32_t pass, uint32
 
 __global__ __launch_bounds__(64) void k_batch_begin(Batch B) {
   if (threadIdx.x < 2u * kBatchBands) B.cursor[threadIdx.x] = 0u;
+  if (B.rays && threadIdx.x == 0u) B.rays->cursor = 0u;
 }
 
 __global__ __launch_bounds__(64) void k_batch_end(Batch B) {
@@ -281,6 +352,21 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
       }
     }
   }
+  // Every band is exhausted: the launch's ray queries fill the rest of the
+  // workgroup's life (the same lane-refill loop as k_query), so workgroups
+  // that finish their tiles early trace rays while the last tiles finish.
+  if (B.rays) {
+    BatchRays* const R = B.rays;
+    const uint32_t n = __builtin_amdgcn_readfirstlane(R->n);
+    if (n) {
+      plain::Ctx C;
+      C.S = S;
+      C.cam = &R->cam;
+      C.grass_taper = nullptr;
+      C.geom_only = true;
+      query_persistent(C, R->rays, R->hits, n, &R->cursor, __builtin_amdgcn_readfirstlane(R->refill_below));
+    }
+  }
 }
 
 // Godray: one workgroup per tile of every view (a flattened grid; the pass is
@@ -334,67 +420,6 @@ __global__ __launch_bounds__(256) void k_composite_taa_batch(Scene S, Batch B) {
   C.cam = V.cam;
   C.grass_taper = nullptr;
   plain::composite_taa_pixel(C, fb, x, y);
-}
-
-// ---------------------------------------------------------------------------
-// Ray queries: one persistent work-stealing launch over all rays.  Each lane
-// owns one ray and advances it one chunk per loop trip; lanes whose ray has
-// finished are refilled with new rays (one wavefront-aggregated atomic on the
-// ray cursor) whenever fewer than `refill_below` lanes are still busy, so a
-// wavefront does not idle behind its longest ray.  Workgroups exit when the
-// ray cursor has passed n and none of their lanes holds a ray.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-
-__global__ __launch_bounds__(64) void k_query_begin(uint32_t* __restrict__ cursor) {
-  if (threadIdx.x == 0) cursor[0] = 0u;
-}
-
-__global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_query(
-    Scene S, const vx_camera* __restrict__ cam, const vx_ray* __restrict__ rays, vx_hit* __restrict__ hits, uint32_t n,
-    uint32_t* __restrict__ cursor, uint32_t refill_below) {
-  plain::Ctx C;
-  C.S = S;
-  C.cam = cam;
-  C.grass_taper = nullptr;
-  C.geom_only = true;
-  plain::QueryRay q;
-  bool active = false, drained = false;
-  uint32_t id = 0u;
-  const unsigned long long below = (1ull << lane_id()) - 1ull;
-  for (;;) {
-    const uint32_t busy = (uint32_t)__popcll(__ballot(active));
-    if (busy < refill_below) {
-      const bool need = !active && !drained;
-      const unsigned long long m = __ballot(need);
-      if (m != 0ull) {
-        uint32_t base = 0u;
-        if (lane_id() == 0u) base = atomicAdd(cursor, (uint32_t)__popcll(m));
-        base = __shfl(base, 0, 64);
-        if (need) {
-          id = base + (uint32_t)__popcll(m & below);
-          if (id < n) {
-            const vx_ray r = rays[id];
-            if (plain::query_begin(C, F3(r.ox, r.oy, r.oz), F3(r.dx, r.dy, r.dz), r.tmax, q)) active = true;
-            else hits[id] = plain::query_result(plain::miss_hitgeom());
-          } else {
-            drained = true;
-          }
-        }
-      }
-    }
-    if (__ballot(active) == 0ull) {
-      if (__ballot(!drained) == 0ull) break;
-      continue;
-    }
-    if (active) {
-      plain::HitGeom h;
-      if (plain::query_step(C, q, h)) {
-        hits[id] = plain::query_result(h);
-        active = false;
-      }
-    }
-  }
 }
 
 }  // namespace

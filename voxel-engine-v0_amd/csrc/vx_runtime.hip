@@ -598,6 +598,12 @@ struct vx_batch {
   hipEvent_t ev[2] = {nullptr, nullptr};
   uint32_t cap = 0;                      // views the device table and ping arrays hold
   std::vector<vxd::BatchView> tab_host;  // the table as last uploaded
+  // Ray queries traced by the fused primary launch (vx_batch_set_rays).
+  vxd::BatchRays* rays_dev = nullptr;
+  vxd::BatchRays* rays_pin = nullptr;  // pinned host copy, uploaded before a launch when it changed
+  hipEvent_t rays_ev = nullptr;        // recorded after the last upload read rays_pin
+  bool rays_dirty = false;
+  uint32_t rays_n = 0;
 };
 
 namespace {
@@ -952,10 +958,43 @@ void vx_batch_destroy(vx_batch* b) {
   dfree(b->table);
   dfree(b->ping);
   dfree(b->cursor);
+  dfree(b->rays_dev);
+  if (b->rays_pin) hipHostFree(b->rays_pin);
+  if (b->rays_ev) hipEventDestroy(b->rays_ev);
   for (auto& e : b->ev)
     if (e) hipEventDestroy(e);
   b->ctx->children.fetch_sub(1);
   delete b;
+}
+
+int vx_batch_set_rays(vx_batch* b, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n, vx_hit* hits_dev) {
+  if (!b || (n && (!scene_cam || !rays_dev || !hits_dev))) return vx_fail(VX_E_INVALID, "vx_batch_set_rays: null argument");
+  DeviceGuard g(b->ctx->device);
+  if (!b->rays_dev) {
+    int rc = dalloc(&b->rays_dev, 1);
+    if (rc) return rc;
+    void* pin = nullptr;
+    if (hipHostMalloc(&pin, sizeof(vxd::BatchRays), hipHostMallocDefault) != hipSuccess || !pin)
+      return vx_fail(VX_E_NOMEM, "vx_batch_set_rays: pinned host allocation");
+    b->rays_pin = static_cast<vxd::BatchRays*>(pin);
+    std::memset(b->rays_pin, 0, sizeof(vxd::BatchRays));
+    if (hipEventCreateWithFlags(&b->rays_ev, hipEventDisableTiming) != hipSuccess)
+      return vx_fail(VX_E_DEVICE, "vx_batch_set_rays: hipEventCreate");
+  }
+  // The last upload must have read the pinned copy before it is rewritten.
+  VX_HIP(hipEventSynchronize(b->rays_ev));
+  vxd::BatchRays& r = *b->rays_pin;
+  std::memset(&r, 0, sizeof(r));
+  if (n) {
+    r.cam = *scene_cam;
+    r.rays = rays_dev;
+    r.hits = hits_dev;
+    r.n = n;
+    r.refill_below = b->ctx->qcfg.refill_below;
+  }
+  b->rays_n = n;
+  b->rays_dirty = true;
+  return VX_OK;
 }
 
 int vx_batch_set_config(vx_batch* b, const vx_batch_config* cfg) {
@@ -993,7 +1032,17 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
     VX_HIP(hipStreamSynchronize(st));
   }
   const vxd::Scene S = make_scene(c, false), SG = make_scene(c, true);
-  const uint32_t persistent = std::min(persistent_groups(b->cfg), std::max(b->B.tiles, 1u));
+  // Ray queries ride in the primary launch: it then keeps every persistent
+  // workgroup (they trace rays once the tiles run out).
+  const bool rays = b->rays_n != 0u && (mask & VX_PASS_PRIMARY);
+  vxd::Batch B = b->B;
+  B.rays = rays ? b->rays_dev : nullptr;
+  const uint32_t persistent = rays ? persistent_groups(b->cfg) : std::min(persistent_groups(b->cfg), std::max(b->B.tiles, 1u));
+  if (rays && b->rays_dirty) {
+    VX_HIP(hipMemcpyAsync(b->rays_dev, b->rays_pin, sizeof(vxd::BatchRays), hipMemcpyHostToDevice, st));
+    VX_HIP(hipEventRecord(b->rays_ev, st));
+    b->rays_dirty = false;
+  }
   if (mask & VX_PASS_PRIMARY) {
     const int rc = refresh_chunk_lights(c, st);
     if (rc) return rc;
@@ -1002,8 +1051,8 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
   for (const vx_view* v : b->views) profile |= (v->profile_flags & 1u) != 0u;
   VX_HIP(hipEventRecord(b->ev[0], st));
   if (b->cfg.use_graph) {
-    const uint64_t key = ((uint64_t)profile << 62) | ((uint64_t)mask << 32) | ((uint64_t)b->cfg.waves_per_simd << 16) |
-                         b->cfg.grab;
+    const uint64_t key = ((uint64_t)rays << 63) | ((uint64_t)profile << 62) | ((uint64_t)mask << 32) |
+                         ((uint64_t)b->cfg.waves_per_simd << 16) | b->cfg.grab;
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
       hipGraph_t graph = nullptr;
@@ -1012,7 +1061,7 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
       VX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
       if (e == hipSuccess) {
-        hipError_t le = vx_launch_batch(S, SG, b->B, mask, persistent, profile, cap);
+        hipError_t le = vx_launch_batch(S, SG, B, mask, persistent, profile, cap);
         e = hipStreamEndCapture(cap, &graph);
         if (e == hipSuccess) e = le;
       }
@@ -1024,7 +1073,7 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
     }
     VX_HIP(hipGraphLaunch(it->second, st));
   } else {
-    VX_HIP(vx_launch_batch(S, SG, b->B, mask, persistent, profile, st));
+    VX_HIP(vx_launch_batch(S, SG, B, mask, persistent, profile, st));
   }
   VX_HIP(hipEventRecord(b->ev[1], st));
   for (uint32_t i = 0; i < n; ++i) {

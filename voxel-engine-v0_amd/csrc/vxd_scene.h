@@ -95,12 +95,25 @@ struct BatchView {
   uint32_t pad;
 };
 constexpr uint32_t kBatchBands = 8;         // one tile band per XCD
+// Ray queries a fused launch traces after the views' primary tiles: device
+// memory owned by the batch, rewritten from a pinned copy before a launch
+// when they change.  The cursor is cleared by the launch's first kernel.
+struct BatchRays {
+  vx_camera cam;                            // the scene camera of the queries
+  const vx_ray* rays;
+  vx_hit* hits;
+  uint32_t n;
+  uint32_t refill_below;                    // lanes refill when fewer than this are busy
+  uint32_t cursor;                          // next ray
+  uint32_t pad;
+};
 struct Batch {
   const BatchView* __restrict__ views;
   uint32_t* __restrict__ ping;              // per view, flipped by the end-of-frame kernel
   uint32_t* __restrict__ cursor;            // [pass][band]: next tile of each band (primary, godray)
   uint32_t n, tiles, rblks, pblks;
   uint32_t grab;                            // tiles taken per atomic
+  BatchRays* rays;                          // null: the launch traces no ray queries
 };
 
 // Workgroup-shared primary candidate list (common.wgsl:422-424).

@@ -229,6 +229,7 @@ def _load() -> C.CDLL:
         "vx_batch_set_config": ([vp, C.POINTER(BatchConfig)], C.c_int),
         "vx_batch_get_config": ([vp, C.POINTER(BatchConfig)], C.c_int),
         "vx_batch_tune": ([vp, u32, u32, C.POINTER(BatchConfig)], C.c_int),
+        "vx_batch_set_rays": ([vp, vp, vp, u32, vp], C.c_int),
         "vx_build_chunk": ([vp, vp, i32, i32, vp, vp, u32, C.POINTER(u32), vp, vp], C.c_int),
         "vx_world_default": ([C.POINTER(WorldParams)], None),
         "vx_world_ground_height": ([C.POINTER(WorldParams), i32, i32], i32),
@@ -268,7 +269,7 @@ EXPORTED = (
     "vx_server_create", "vx_server_destroy", "vx_server_submit", "vx_server_poll", "vx_server_edit",
     "vx_server_wait_edits", "vx_server_drain", "vx_server_close_stream", "vx_server_get_stats",
     "vx_query_rays", "vx_query_timing", "vx_query_set_config", "vx_query_get_config", "vx_query_tune", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
-    "vx_batch_set_config", "vx_batch_get_config", "vx_batch_tune", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
+    "vx_batch_set_config", "vx_batch_get_config", "vx_batch_tune", "vx_batch_set_rays", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
     "vx_host_world_build", "vx_host_world_free", "vx_host_world_get", "vx_host_world_edit", "vx_camera_build",
     "vx_clipmap_build", "vx_clipmap_state_create", "vx_clipmap_state_destroy", "vx_clipmap_invalidate",
     "vx_clipmap_update", "vx_clipmap_fill_rects", "vx_write_clipmap_rects", "vx_read_clipmap", "vx_read_region",
@@ -970,6 +971,15 @@ class Batch:
         cfg = BatchConfig()
         _check(LIB.vx_batch_tune(self._h, mask, iters, C.byref(cfg)), "vx_batch_tune")
         return cfg
+
+    def set_rays(self, scene_cam: Camera | None, rays: "DeviceArray | None", hits: "DeviceArray | None",
+                 n: int) -> None:
+        """vx_batch_set_rays: later renders with the primary pass also trace
+        these n device-resident rays in the same launch (n = 0 detaches).
+        The arrays are kept referenced until they are replaced."""
+        self._rays = (rays, hits) if n else None
+        _check(LIB.vx_batch_set_rays(self._h, C.byref(scene_cam) if scene_cam is not None else None,
+                                     rays.ptr if n else None, n, hits.ptr if n else None), "vx_batch_set_rays")
 
     def close(self) -> None:
         if self._h:
