@@ -351,7 +351,9 @@ typedef struct vx_batch_config {
     uint32_t waves_per_simd;  /* persistent workgroups = CUs * 4 SIMDs * this (default 6) */
     uint32_t grab;            /* tiles taken per work-stealing atomic (default 1) */
     uint32_t use_graph;       /* replay the pass sequence from a captured hipGraph (default 1) */
-    uint32_t pad;
+    uint32_t tile_shape;      /* primary execution tile: 0 8x8 (default), 1 16x4, 2 32x2; each
+                                 pixel still traces with its 8x8 tile's candidate list, which
+                                 shapes 1 and 2 build in a pre-pass (bit-identical outputs) */
 } vx_batch_config;
 int  vx_batch_create(vx_ctx* ctx, vx_view* const* views, uint32_t n, vx_batch** out);
 void vx_batch_destroy(vx_batch* batch);
@@ -362,8 +364,9 @@ int  vx_batch_timing(vx_batch* batch, float* ms);
 int  vx_batch_set_config(vx_batch* batch, const vx_batch_config* cfg);
 int  vx_batch_get_config(const vx_batch* batch, vx_batch_config* cfg);
 /* Autotuner: times the passes in pass_mask (history is not advanced) for
- * waves_per_simd in {4, 6, 8} x grab in {1, 2}, `iters` runs each,
- * keeps the fastest configuration and returns it in *chosen (may be NULL). */
+ * tile_shape in {8x8, 16x4, 32x2} x waves_per_simd in {4, 8} x grab in {1, 2},
+ * `iters` runs each, keeps the fastest configuration and returns it in
+ * *chosen (may be NULL). */
 int  vx_batch_tune(vx_batch* batch, uint32_t pass_mask, uint32_t iters, vx_batch_config* chosen);
 /* No counterpart in the reference: ray batches fused into a batch's launch.
  * Each later vx_batch_render whose pass_mask holds VX_PASS_PRIMARY also

@@ -2,8 +2,10 @@
 fused work-stealing launches give, frame after frame, exactly the outputs of
 rendering each view on its own with vx_render (which the parity tests tie to
 the oracle).  Covers hipGraph replay and direct launches, several autotuner
-configurations, a view rendered on its own between batch frames (history
-ping-pong resynchronised), and tuning runs that must leave history as it was."""
+configurations including the 16x4 and 32x2 execution tiles (each pixel with
+its 8x8 tile's candidate list), a view rendered on its own between batch
+frames (history ping-pong resynchronised), and tuning runs that must leave
+history as it was."""
 from __future__ import annotations
 
 import numpy as np
@@ -44,7 +46,9 @@ def test_batch_equals_single_views(gpu):
     sets = {}
     for name, cfg in (("graph", dict(waves_per_simd=6, grab=1, use_graph=True)),
                       ("direct", dict(waves_per_simd=4, grab=2, use_graph=False)),
-                      ("wide", dict(waves_per_simd=12, grab=4, use_graph=True))):
+                      ("wide", dict(waves_per_simd=12, grab=4, use_graph=True)),
+                      ("16x4", dict(waves_per_simd=4, grab=1, use_graph=True, tile_shape=1)),
+                      ("32x2", dict(waves_per_simd=8, grab=2, use_graph=False, tile_shape=2))):
         views = [gpu.View(ctx, p, r) for p, r in SIZES]
         batch = gpu.Batch(ctx, views)
         batch.set_config(**cfg)

@@ -252,21 +252,24 @@ __device__ __forceinline__ uint32_t batch_view_of(const Batch& B, uint32_t item,
   while (lo < hi) {
     const uint32_t mid = (lo + hi + 1u) >> 1;
     const BatchView& V = B.views[mid];
-    const uint32_t base = which == 0 ? V.tile_base : (which == 1 ? V.rblk_base : V.pblk_base);
+    const uint32_t base = which == 0 ? V.tile_base
+                        : which == 1 ? V.rblk_base
+                        : which == 2 ? V.pblk_base
+                                     : V.xt_base[which - 3];  // 3, 4: execution tiles of shapes 1, 2
     if (base <= item) lo = mid;
     else hi = mid - 1u;
   }
   return lo;
 }
 
-// Next run of up to B.grab tiles for this workgroup, or false when every band
-// is exhausted.  `band` is the band this workgroup is currently draining; it
+// Next run of up to B.grab of the `total` tiles for this workgroup, or false
+// when every band is exhausted.  `band` is the band this workgroup is currently draining; it
 // starts at the workgroup's XCD and advances when the band runs dry.
-__device__ __forceinline__ bool batch_take(const Batch& B, uint32_t pass, uint32_t& band, uint32_t& steps, uint32_t& t0,
-                                           uint32_t& t1) {
-  const uint32_t per = (B.tiles + kBatchBands - 1u) / kBatchBands;
+__device__ __forceinline__ bool batch_take(const Batch& B, uint32_t total, uint32_t pass, uint32_t& band, uint32_t& steps,
+                                           uint32_t& t0, uint32_t& t1) {
+  const uint32_t per = (total + kBatchBands - 1u) / kBatchBands;
   while (steps < kBatchBands) {
-    const uint32_t lo = band * per, hi = min(B.tiles, lo + per);
+    const uint32_t lo = band * per, hi = min(total, lo + per);
     uint32_t got = 0u;
     if (threadIdx.x == 0) got = atomicAdd(B.cursor + pass * kBatchBands + band, B.grab);
     got = __builtin_amdgcn_readfirstlane(got);
@@ -290,10 +293,58 @@ __global__ __launch_bounds__(64) void k_batch_end(Batch B) {
   for (uint32_t v = threadIdx.x; v < B.n; v += 64u) B.ping[v] ^= 1u;
 }
 
+// The 8x8 tile's candidate chunk list into L over the whole wavefront (the
+// phases of plain::gather_*), or on one lane when a probe ray visits more
+// cells than the phases hold.  The workgroup is one wavefront, and every
+// branch below is uniform over it.
+__device__ __forceinline__ void tile_gather(const plain::Ctx& C, TileLDS& L, uint32_t wgx, uint32_t wgy) {
+  plain::gather_walk(C, L, wgx, wgy, threadIdx.x);
+  __syncthreads();
+  if (plain::gather_overflow(L)) {
+    if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
+  } else {
+    plain::gather_load(C, L, threadIdx.x);
+    __syncthreads();
+    plain::gather_first(L, threadIdx.x);
+    __syncthreads();
+    plain::gather_place(L, threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 0) plain::gather_finish(L);
+  }
+  __syncthreads();
+}
+
+static_assert(PRIMARY_MAX_TILE_CHUNKS + 1u <= kListWords, "a list holds its count and the cached slots");
+
+// The candidate lists of every 8x8 tile of the batch's views, for the
+// execution tile shapes other than 8x8: one workgroup per tile (XCD-banded),
+// the same gather as the primary kernel's, stored as [count, slots...].
+__global__ __launch_bounds__(64) void k_tile_lists(Scene S, Batch B) {
+  __shared__ TileLDS L;
+  const uint32_t t = xcd_tile(blockIdx.x, B.tiles);
+  const uint32_t v = batch_view_of(B, t, 0);
+  const BatchView& V = B.views[v];
+  const uint32_t tile = t - V.tile_base;
+  plain::Ctx C;
+  C.S = S;
+  C.cam = V.cam;
+  C.grass_taper = nullptr;
+  tile_gather(C, L, tile % V.tiles_x, tile / V.tiles_x);
+  uint32_t* out = B.lists + (size_t)t * kListWords;
+  const uint32_t n = L.cached;
+  if (threadIdx.x == 0) out[0] = n;
+  if (threadIdx.x < n) out[1u + threadIdx.x] = L.slots[threadIdx.x];
+}
+
 // PROF: the launch carries a view whose camera has profile_flags & 1; the
 // product launch (PROF = false) has no profiling code at all.
-template <bool PROF>
+//
+// SHAPE: the execution tile (0: 8x8, 1: 16x4, 2: 32x2).  With 8x8 the
+// wavefront builds its tile's candidate list in LDS; with the others each lane
+// traces with its own 8x8 tile's list from the pre-pass (k_tile_lists).
+template <bool PROF, uint32_t SHAPE>
 __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scene S, Batch B) {
+  constexpr uint32_t TW = 8u << SHAPE, TH = 64u / TW;
   __shared__ TileLDS L;
   // One deferral region per workgroup (the launchers never start more
   // workgroups than there are regions).
@@ -301,14 +352,13 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
   uint4* const defer = S.defer + (size_t)blockIdx.x * (kDeferEntries * 64u) + threadIdx.x;
   if (threadIdx.x < 12u) L.grass_taper[threadIdx.x] = plain::grass_taper_entry(threadIdx.x);
   if (threadIdx.x < 5u) L.prof[threadIdx.x] = 0u;
+  const uint32_t total = SHAPE == 0u ? B.tiles : B.xtiles[SHAPE - 1u];
   uint32_t band = blockIdx.x % kBatchBands, steps = 0u, t0 = 0u, t1 = 0u;
-  while (batch_take(B, 0u, band, steps, t0, t1)) {
+  while (batch_take(B, total, 0u, band, steps, t0, t1)) {
     for (uint32_t t = t0; t < t1; ++t) {
-      const uint32_t v = batch_view_of(B, t, 0);
+      const uint32_t v = batch_view_of(B, t, SHAPE == 0u ? 0 : (int)(2u + SHAPE));
       const BatchView& V = B.views[v];
       const FrameBuffers& fb = V.fb[B.ping[v] & 1u];
-      const uint32_t tile = t - V.tile_base;
-      const uint32_t wgx = tile % V.tiles_x, wgy = tile / V.tiles_x;
       plain::Ctx C;
       C.S = S;
       C.cam = V.cam;
@@ -320,28 +370,24 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
       const bool prof = PROF && (V.cam->profile_flags & 1u) != 0u;
       C.prof = prof ? L.prof : nullptr;
       __syncthreads();  // the previous tile's lanes are done with L
-      // The tile's candidate chunk list over the whole wavefront (the phases
-      // of plain::gather_*), or on one lane when a probe ray visits more cells
-      // than the phases hold.  The workgroup is one wavefront, and every
-      // branch below is uniform over it.
-      plain::gather_walk(C, L, wgx, wgy, threadIdx.x);
-      __syncthreads();
-      if (plain::gather_overflow(L)) {
-        if (threadIdx.x == 0) plain::tile_build(C, L, wgx, wgy);
+      if constexpr (SHAPE == 0u) {
+        const uint32_t tile = t - V.tile_base;
+        const uint32_t wgx = tile % V.tiles_x, wgy = tile / V.tiles_x;
+        tile_gather(C, L, wgx, wgy);
+        // Every lane runs the pixel pass (those outside the image only help
+        // with the deferred grass tests, which need the whole wavefront).
+        const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
+        plain::primary_pixel(C, plain::LdsList{L}, fb, gx, gy, gx < fb.rw && gy < fb.rh, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
       } else {
-        plain::gather_load(C, L, threadIdx.x);
-        __syncthreads();
-        plain::gather_first(L, threadIdx.x);
-        __syncthreads();
-        plain::gather_place(L, threadIdx.x);
-        __syncthreads();
-        if (threadIdx.x == 0) plain::gather_finish(L);
+        const uint32_t xt = t - V.xt_base[SHAPE - 1u];
+        const uint32_t xtx = V.xt_x[SHAPE - 1u];
+        const uint32_t gx = (xt % xtx) * TW + (threadIdx.x % TW), gy = (xt / xtx) * TH + (threadIdx.x / TW);
+        // this pixel's 8x8 tile (clamped into the view for lanes past its edge)
+        const uint32_t ty = V.tiles / V.tiles_x;
+        const uint32_t lt = min(gy >> 3, ty - 1u) * V.tiles_x + min(gx >> 3, V.tiles_x - 1u);
+        const plain::LaneList lst{B.lists + (size_t)(V.tile_base + lt) * kListWords};
+        plain::primary_pixel(C, lst, fb, gx, gy, gx < fb.rw && gy < fb.rh, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
       }
-      __syncthreads();
-      // Every lane runs the pixel pass (those outside the image only help with
-      // the deferred grass tests, which need the whole wavefront).
-      const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
-      plain::primary_pixel(C, L, fb, gx, gy, gx < fb.rw && gy < fb.rh, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
       if (prof) {
         __syncthreads();  // every lane's LDS increments are in
         if (threadIdx.x < 5u) {
@@ -464,8 +510,17 @@ hipError_t vx_launch_batch(const vxd::Scene& S, const vxd::Scene& S_godray, cons
   hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, B);
   if (mask & VX_PASS_PRIMARY) {
     const uint32_t groups = std::min(persistent, S.n_defer_regions);
-    if (profile) hipLaunchKernelGGL(k_primary_batch<true>, dim3(groups), dim3(64), 0, st, S, B);
-    else hipLaunchKernelGGL(k_primary_batch<false>, dim3(groups), dim3(64), 0, st, S, B);
+    if (B.shape != 0u && B.tiles) hipLaunchKernelGGL(k_tile_lists, dim3(B.tiles), dim3(64), 0, st, S, B);
+    if (B.shape == 1u) {
+      if (profile) hipLaunchKernelGGL((k_primary_batch<true, 1u>), dim3(groups), dim3(64), 0, st, S, B);
+      else hipLaunchKernelGGL((k_primary_batch<false, 1u>), dim3(groups), dim3(64), 0, st, S, B);
+    } else if (B.shape == 2u) {
+      if (profile) hipLaunchKernelGGL((k_primary_batch<true, 2u>), dim3(groups), dim3(64), 0, st, S, B);
+      else hipLaunchKernelGGL((k_primary_batch<false, 2u>), dim3(groups), dim3(64), 0, st, S, B);
+    } else {
+      if (profile) hipLaunchKernelGGL((k_primary_batch<true, 0u>), dim3(groups), dim3(64), 0, st, S, B);
+      else hipLaunchKernelGGL((k_primary_batch<false, 0u>), dim3(groups), dim3(64), 0, st, S, B);
+    }
   }
   if (mask & VX_PASS_LOCAL_TAA) hipLaunchKernelGGL(k_local_taa_batch, dim3(B.rblks), dim3(256), 0, st, B);
   if (mask & VX_PASS_GODRAY) hipLaunchKernelGGL(k_godray_batch, dim3(B.tiles), dim3(64), 0, st, S_godray, B);
@@ -483,8 +538,8 @@ hipError_t vx_launch_ws_begin(const vxd::Batch& B, hipStream_t st) {
 hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32_t persistent, bool profile,
                                 hipStream_t st) {
   const uint32_t groups = std::min(persistent, S.n_defer_regions);
-  if (profile) hipLaunchKernelGGL(k_primary_batch<true>, dim3(groups), dim3(64), 0, st, S, B);
-  else hipLaunchKernelGGL(k_primary_batch<false>, dim3(groups), dim3(64), 0, st, S, B);
+  if (profile) hipLaunchKernelGGL((k_primary_batch<true, 0u>), dim3(groups), dim3(64), 0, st, S, B);
+  else hipLaunchKernelGGL((k_primary_batch<false, 0u>), dim3(groups), dim3(64), 0, st, S, B);
   return hipGetLastError();
 }
 

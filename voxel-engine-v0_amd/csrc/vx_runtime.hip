@@ -604,6 +604,10 @@ struct vx_batch {
   hipEvent_t rays_ev = nullptr;        // recorded after the last upload read rays_pin
   bool rays_dirty = false;
   uint32_t rays_n = 0;
+  // Candidate lists of every 8x8 tile for the execution tile shapes other
+  // than 8x8 (allocated on the first such render, grown with the views).
+  uint32_t* lists = nullptr;
+  uint32_t lists_cap = 0;  // tiles the list buffer holds
 };
 
 namespace {
@@ -665,9 +669,13 @@ vxd::FrameBuffers view_buffers(const vx_view* v, int ping, bool hits) {
 // capacity; graphs captured for the old table are dropped.
 // Table entries of a fused launch over `views` (in table order) and the
 // launch totals: tiles, render blocks, present blocks.
+// The views' table entries and their work item totals (8x8 tiles, render
+// and present blocks); xtiles, if given, receives the execution tiles of
+// shapes 16x4 and 32x2.
 void batch_entries(vx_view* const* views, uint32_t n, vxd::BatchView* tab, uint32_t& tiles, uint32_t& rblks,
-                   uint32_t& pblks) {
+                   uint32_t& pblks, uint32_t* xtiles = nullptr) {
   tiles = rblks = pblks = 0;
+  uint32_t xt[2] = {0u, 0u};
   for (uint32_t i = 0; i < n; ++i) {
     const vx_view* v = views[i];
     vxd::BatchView& e = tab[i];
@@ -687,7 +695,14 @@ void batch_entries(vx_view* const* views, uint32_t n, vxd::BatchView* tab, uint3
     tiles += e.tiles;
     rblks += e.rblks;
     pblks += e.pblks;
+    for (uint32_t k = 0; k < 2u; ++k) {  // 16x4, 32x2
+      const uint32_t tw = 16u << k, th = 64u / tw;
+      e.xt_x[k] = (v->rw + tw - 1u) / tw;
+      e.xt_base[k] = xt[k];
+      xt[k] += e.xt_x[k] * ((v->rh + th - 1u) / th);
+    }
   }
+  if (xtiles) xtiles[0] = xt[0], xtiles[1] = xt[1];
 }
 
 bool larger_render(const vx_view* a, const vx_view* c) { return (uint64_t)a->rw * a->rh > (uint64_t)c->rw * c->rh; }
@@ -699,8 +714,8 @@ int batch_bind(vx_batch* b, vx_view* const* views, uint32_t n) {
   b->tab_host.resize(n);
   b->gens.clear();
   b->ping_host.clear();
-  uint32_t tiles = 0, rblks = 0, pblks = 0;
-  batch_entries(b->views.data(), n, b->tab_host.data(), tiles, rblks, pblks);
+  uint32_t tiles = 0, rblks = 0, pblks = 0, xtiles[2] = {0u, 0u};
+  batch_entries(b->views.data(), n, b->tab_host.data(), tiles, rblks, pblks, xtiles);
   for (const vx_view* v : b->views) {
     b->gens.push_back(v->gen);
     b->ping_host.push_back((uint32_t)v->ping);
@@ -728,6 +743,8 @@ int batch_bind(vx_batch* b, vx_view* const* views, uint32_t n) {
   b->B.rblks = rblks;
   b->B.pblks = pblks;
   b->B.grab = b->cfg.grab;
+  b->B.xtiles[0] = xtiles[0];
+  b->B.xtiles[1] = xtiles[1];
   return VX_OK;
 }
 
@@ -959,6 +976,7 @@ void vx_batch_destroy(vx_batch* b) {
   dfree(b->ping);
   dfree(b->cursor);
   dfree(b->rays_dev);
+  dfree(b->lists);
   if (b->rays_pin) hipHostFree(b->rays_pin);
   if (b->rays_ev) hipEventDestroy(b->rays_ev);
   for (auto& e : b->ev)
@@ -999,8 +1017,9 @@ int vx_batch_set_rays(vx_batch* b, const vx_camera* scene_cam, const vx_ray* ray
 
 int vx_batch_set_config(vx_batch* b, const vx_batch_config* cfg) {
   if (!b || !cfg) return vx_fail(VX_E_INVALID, "vx_batch_set_config: null argument");
-  if (cfg->waves_per_simd < 1u || cfg->waves_per_simd > 16u || cfg->grab < 1u || cfg->grab > 64u)
-    return vx_fail(VX_E_INVALID, "vx_batch_set_config: waves_per_simd in 1..16, grab in 1..64");
+  if (cfg->waves_per_simd < 1u || cfg->waves_per_simd > 16u || cfg->grab < 1u || cfg->grab > 64u ||
+      cfg->tile_shape >= vxd::kTileShapes)
+    return vx_fail(VX_E_INVALID, "vx_batch_set_config: waves_per_simd in 1..16, grab in 1..64, tile_shape in 0..2");
   b->cfg = *cfg;
   b->B.grab = cfg->grab;
   return VX_OK;
@@ -1037,7 +1056,18 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
   const bool rays = b->rays_n != 0u && (mask & VX_PASS_PRIMARY);
   vxd::Batch B = b->B;
   B.rays = rays ? b->rays_dev : nullptr;
-  const uint32_t persistent = rays ? persistent_groups(b->cfg) : std::min(persistent_groups(b->cfg), std::max(b->B.tiles, 1u));
+  B.shape = b->cfg.tile_shape;
+  if (B.shape != 0u && b->lists_cap < B.tiles) {
+    dfree(b->lists);
+    b->lists = nullptr;
+    b->lists_cap = 0;
+    drop_graphs(b);  // captured launches hold the old buffer
+    if (int rc = dalloc(&b->lists, (size_t)B.tiles * vxd::kListWords)) return rc;
+    b->lists_cap = B.tiles;
+  }
+  B.lists = b->lists;
+  const uint32_t exec_tiles = B.shape == 0u ? B.tiles : B.xtiles[B.shape - 1u];
+  const uint32_t persistent = rays ? persistent_groups(b->cfg) : std::min(persistent_groups(b->cfg), std::max(exec_tiles, 1u));
   if (rays && b->rays_dirty) {
     VX_HIP(hipMemcpyAsync(b->rays_dev, b->rays_pin, sizeof(vxd::BatchRays), hipMemcpyHostToDevice, st));
     VX_HIP(hipEventRecord(b->rays_ev, st));
@@ -1052,7 +1082,7 @@ int vx_batch_render(vx_batch* b, uint32_t mask, void* stream) {
   VX_HIP(hipEventRecord(b->ev[0], st));
   if (b->cfg.use_graph) {
     const uint64_t key = ((uint64_t)rays << 63) | ((uint64_t)profile << 62) | ((uint64_t)mask << 32) |
-                         ((uint64_t)b->cfg.waves_per_simd << 16) | b->cfg.grab;
+                         ((uint64_t)b->cfg.tile_shape << 24) | ((uint64_t)b->cfg.waves_per_simd << 16) | b->cfg.grab;
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
       hipGraph_t graph = nullptr;
@@ -1163,11 +1193,12 @@ int vx_batch_tune(vx_batch* b, uint32_t mask, uint32_t iters, vx_batch_config* c
   const vx_batch_config keep = b->cfg;
   vx_batch_config best = keep;
   float best_ms = 1e30f;
-  const uint32_t waves[] = {4u, 6u, 8u};
+  const uint32_t waves[] = {4u, 8u};
   const uint32_t grabs[] = {1u, 2u};
+  for (uint32_t shape = 0; shape < vxd::kTileShapes; ++shape)
   for (uint32_t w : waves)
     for (uint32_t gsz : grabs) {
-      vx_batch_config cfg{w, gsz, keep.use_graph, 0u};
+      vx_batch_config cfg{w, gsz, keep.use_graph, shape};
       int rc = vx_batch_set_config(b, &cfg);
       if (rc) return rc;
       float total = 0.0f;

@@ -229,5 +229,12 @@ VXD float wave_read(float v, uint32_t src) { return __shfl(v, (int)src, 64); }
 // Lane `dst` receives v (each active lane sends to one lane).
 VXD uint32_t wave_send(uint32_t dst, uint32_t v) { return (uint32_t)__builtin_amdgcn_ds_permute((int)(dst << 2), (int)v); }
 VXD uint32_t popc64(uint64_t x) { return (uint32_t)__popcll(x); }
+// The largest v < 32 over the active lanes (ballots only, so any exec mask).
+VXD uint32_t wave_max_small(uint32_t v) {
+  uint32_t lo = 0u;
+  for (uint32_t step = 16u; step > 0u; step >>= 1u)
+    if (wave_any(v >= lo + step)) lo += step;
+  return lo;
+}
 
 }  // namespace vxd

@@ -92,6 +92,9 @@ struct BatchView {
   uint32_t tiles_x, tiles, tile_base;       // 8x8 tiles at render resolution
   uint32_t rblk_x, rblks, rblk_base;        // 16x16 blocks at render resolution
   uint32_t pblk_x, pblks, pblk_base;        // 16x16 blocks at present resolution
+  // Primary execution tiles of the other shapes (16x4, 32x2): per row and
+  // the view's first index in the batch's execution tile space of that shape.
+  uint32_t xt_x[2], xt_base[2];
   uint32_t pad;
 };
 constexpr uint32_t kBatchBands = 8;         // one tile band per XCD
@@ -107,6 +110,11 @@ struct BatchRays {
   uint32_t cursor;                          // next ray
   uint32_t pad;
 };
+// Primary execution tile shapes: 8x8 (the candidate list's own tile), 16x4
+// and 32x2.  Whatever the shape, each pixel traces with the candidate list of
+// its 8x8 tile; shapes other than 8x8 take the lists from a pre-pass.
+constexpr uint32_t kTileShapes = 3;
+constexpr uint32_t kListWords = 32;         // per 8x8 tile: the count, then up to 24 chunk slots
 struct Batch {
   const BatchView* __restrict__ views;
   uint32_t* __restrict__ ping;              // per view, flipped by the end-of-frame kernel
@@ -114,6 +122,9 @@ struct Batch {
   uint32_t n, tiles, rblks, pblks;
   uint32_t grab;                            // tiles taken per atomic
   BatchRays* rays;                          // null: the launch traces no ray queries
+  uint32_t shape;                           // primary execution tile shape (0: 8x8, 1: 16x4, 2: 32x2)
+  uint32_t xtiles[2];                       // execution tiles of shapes 1 and 2 over all views
+  uint32_t* __restrict__ lists;             // shapes 1, 2: [8x8 tile][kListWords] from the pre-pass
 };
 
 // Workgroup-shared primary candidate list (common.wgsl:422-424).

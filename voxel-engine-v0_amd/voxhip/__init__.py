@@ -136,10 +136,15 @@ class Pose(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("waves_per_simd", C.c_uint32), ("grab", C.c_uint32), ("use_graph", C.c_uint32), ("pad", C.c_uint32)]
+    _fields_ = [("waves_per_simd", C.c_uint32), ("grab", C.c_uint32), ("use_graph", C.c_uint32),
+                ("tile_shape", C.c_uint32)]
 
     def as_dict(self) -> dict:
-        return {"waves_per_simd": self.waves_per_simd, "grab": self.grab, "use_graph": self.use_graph}
+        return {"waves_per_simd": self.waves_per_simd, "grab": self.grab, "use_graph": self.use_graph,
+                "tile_shape": TILE_SHAPES[self.tile_shape] if self.tile_shape < len(TILE_SHAPES) else self.tile_shape}
+
+
+TILE_SHAPES = ("8x8", "16x4", "32x2")  # vx_batch_config.tile_shape
 
 
 class QueryConfig(C.Structure):
@@ -963,8 +968,8 @@ class Batch:
         _check(LIB.vx_batch_get_config(self._h, C.byref(cfg)), "vx_batch_get_config")
         return cfg
 
-    def set_config(self, waves_per_simd: int = 6, grab: int = 1, use_graph: bool = True) -> None:
-        cfg = BatchConfig(waves_per_simd, grab, 1 if use_graph else 0, 0)
+    def set_config(self, waves_per_simd: int = 6, grab: int = 1, use_graph: bool = True, tile_shape: int = 0) -> None:
+        cfg = BatchConfig(waves_per_simd, grab, 1 if use_graph else 0, tile_shape)
         _check(LIB.vx_batch_set_config(self._h, C.byref(cfg)), "vx_batch_set_config")
 
     def tune(self, mask: int = PASS_FRAME, iters: int = 2) -> BatchConfig:
