@@ -70,6 +70,7 @@ def parse():
 
 TRAFFIC_FILE = REPO / "profiles" / "k_primary_batch_traffic.md"
 PROFILE_FILE = REPO / "profiles" / "k_primary_batch_4k_flight.md"
+QUERY_PROFILE_FILE = REPO / "profiles" / "k_query_32x4x32.md"
 VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock: 39.3 T non-packed lane-ops/s
 
 
@@ -119,18 +120,32 @@ def measured_traffic(W, H):
     return int(val.strip()), None
 
 
-def profiled_valu():
-    """SQ/TCC counter figures of k_primary_batch from the committed rocprofv3
-    summary of the same 4K workload and this kernel build, or (None, reason)."""
-    keys, why = profile_of_this_build(PROFILE_FILE)
+def profiled_valu(path: Path = PROFILE_FILE):
+    """SQ/TCC counter figures of a kernel from a committed rocprofv3 summary of
+    the same workload and this kernel build (by default k_primary_batch at 4K),
+    or (None, reason)."""
+    keys, why = profile_of_this_build(path)
     if keys is None:
         return None, why
     try:
         return {"valu_insts_per_launch": float(keys["valu_insts_per_launch"]),
                 "valu_busy": float(keys["valu_busy"]), "valu_lane_utilisation": float(keys["valu_lane_utilisation"]),
-                "l2_hit_rate": float(keys["l2_hit_rate"]), "source": str(PROFILE_FILE.relative_to(REPO))}, None
+                "l2_hit_rate": float(keys["l2_hit_rate"]), "source": str(path.relative_to(REPO))}, None
     except (KeyError, ValueError) as e:
-        return None, f"{PROFILE_FILE.relative_to(REPO)}: {e}"
+        return None, f"{path.relative_to(REPO)}: {e}"
+
+
+def query_traffic(n):
+    """HBM bytes of the fused k_query launch over n rays, from the committed
+    counter summary of this kernel build (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated), or (None, reason)."""
+    keys, why = profile_of_this_build(QUERY_PROFILE_FILE)
+    if keys is None:
+        return None, why
+    val, _, rays = keys.get("traffic_bytes_per_launch", "").partition("@")
+    if rays.strip() != f"{n} rays":
+        return None, f"no traffic figure for {n} rays"
+    return int(val.strip()), None
 
 
 def copy_calibration(mib):
@@ -171,6 +186,8 @@ def main():
                                                        device_build=True),
                 "serving_1080p": lambda: serve_1080p(vx, scenes, 4.0)}
         print(json.dumps({a.only: legs[a.only]()}), flush=True)
+        if a.copy_calib:
+            copy_calibration(a.copy_calib)
         return
 
     W, H = (int(v) for v in a.res.lower().split("x"))
@@ -651,14 +668,44 @@ def ray_queries(vx, scenes, n_batches):
     ctx.sync()
     t_plain = time.perf_counter() - t
     hits = d_hits.download(np.empty(n, vx.HIT_DTYPE))
+    # The query's roofline: the counting forms of the same rays (one ray per
+    # lane, per-lane load counters) give the reference algorithm's bytes and the
+    # bytes k_query loads; both over the fused launch's HIP-event time.
+    c_ref, p_ref = vx.query_count(ctx, cam, d_rays, d_hits, n)
+    c_exe, _ = vx.query_count(ctx, cam, d_rays, d_hits, n, executed=True)
     d_rays.close()
     d_hits.close()
+    b_ref, b_exe = vx.algorithmic_bytes(c_ref), vx.algorithmic_bytes(c_exe)
+    achieved = b_ref / (gpu_fused_ms * 1e-3) / 1e9
+    achieved_x = b_exe / (gpu_fused_ms * 1e-3) / 1e9
+    traffic, traffic_note = query_traffic(n)
+    prof, prof_note = profiled_valu(QUERY_PROFILE_FILE)
+    roofline = {"bound": "hbm", "kernel": "k_query", "kernel_build": kernel_build(), "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "algorithmic_bytes_per_launch": int(b_ref), "algorithmic_bytes_per_ray": round(b_ref / n, 1),
+                "loads_per_ray": {k: round(v / n, 3) for k, v in c_ref.items()},
+                "executed": {"bytes_per_launch": int(b_exe), "bytes_per_ray": round(b_exe / n, 1),
+                             "achieved": round(achieved_x, 2), "frac": round(achieved_x / HBM_PEAK_GBS, 5)},
+                "profiled": prof}
+    if traffic is None:
+        roofline["traffic_note"] = traffic_note
+    if prof is None:
+        roofline["profiled_note"] = prof_note
+    else:
+        # the VALU roof as for the primary pass: lane-operations executed per second
+        lane_ops = prof["valu_insts_per_launch"] * 64.0 * prof["valu_lane_utilisation"]
+        issue_ops = prof["valu_insts_per_launch"] * 64.0
+        roofline["valu"] = {"bound": "valu", "achieved": round(lane_ops / (gpu_fused_ms * 1e-3) / 1e12, 3),
+                            "peak": round(VALU_PEAK_LANE_OPS / 1e12, 3), "unit": "T lane-op/s",
+                            "frac": round(lane_ops / (gpu_fused_ms * 1e-3) / VALU_PEAK_LANE_OPS, 4),
+                            "issue_frac": round(issue_ops / (gpu_fused_ms * 1e-3) / VALU_PEAK_LANE_OPS, 4),
+                            "lane_ops_per_ray": round(lane_ops / n, 1)}
     return {"workload": f"{n_batches} batches, log-uniform 1e3..1e6 rays (seed 11), 32x4x32-chunk world, origins uniform "
                         f"in its AABB, directions uniform, tmax 51.2 m", "rays": n, "chunks": hw.chunk_count,
             "world_build_s": round(build_s, 2), "fused_rays_per_s": round(n / t_fused, 1),
             "fused_gpu_ms": round(gpu_fused_ms, 3), "plain_rays_per_s": round(n / t_plain, 1),
             "speedup": round(t_plain / t_fused, 3), "hit_fraction": round(float(np.mean(hits["kind"] == 1)), 4),
-            "tuned": tuned}
+            "tuned": tuned, "roofline": roofline}
 
 
 def serve(vx, scenes, seconds, rate=400.0, edit_rate=1000.0, n_streams=32, budget=64 << 30, target_ms=100.0,

@@ -324,6 +324,17 @@ int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_de
                   uint32_t n, vx_hit* hits_dev, void* stream);
 /* GPU milliseconds of the last vx_query_rays (waits for it). */
 int vx_query_timing(vx_ctx* ctx, float* ms);
+/* Algorithmic load counts of a query over rays_dev[0, n) (no reference
+ * counterpart: the measurement form of vx_query_rays, as vx_view_counters is
+ * of the primary pass).  Each ray is traced alone on one lane with per-lane
+ * counters: the reference algorithm's loads (executed = 0) or the loads
+ * vx_query_rays makes (executed = 1).  out[9] as vx_view_counters: elements
+ * loaded from [grid, meta, macro, node, rope, colinfo, clip], then bytes of
+ * [ray records read, hit records written]; parts (may be NULL) as
+ * vx_view_count_parts.  hits_dev receives the hits vx_query_rays writes.
+ * Synchronous: waits for the context's stream before and after. */
+int vx_query_count(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n,
+                   vx_hit* hits_dev, int executed, uint64_t out[9], uint64_t parts[VX_COUNT_PARTS]);
 typedef struct vx_query_config {
     uint32_t waves_per_simd;  /* persistent workgroups = CUs * 4 SIMDs * this (default 8) */
     uint32_t refill_below;    /* refill idle lanes when fewer lanes than this are busy (default 48) */

@@ -71,6 +71,43 @@ def test_query_boundary_rays_match_oracle(gpu):
     ctx.close()
 
 
+def test_query_counted_loads_match_oracle(gpu):
+    """vx_query_count, the counting form behind the query roofline's
+    algorithmic bytes: every load counter equal to the oracle's own counts for
+    the same rays, the ray and hit records counted once each, the hits equal
+    to vx_query_rays', and the executed loads no more than the reference's."""
+    from oracle import oracle as O
+    from voxhip import scenes
+
+    sc, hw = host_world("c1")
+    ctx = make_context(gpu, hw)
+    cam = _scene_cam(gpu, sc, hw)
+    rays = scenes.query_rays(np.random.default_rng(5), 20_000, sc.grid_origin, sc.grid_dims, tmax_m=51.2)
+    n = len(rays)
+    clip, heights = worlds.empty_clipmap(gpu)
+    oc = O.Counts()
+    ref = O.query_rays(O.World.from_host(hw, clip, heights), cam, rays, gpu.HIT_DTYPE, counts=oc)
+    d_rays = gpu.DeviceArray.from_numpy(rays)
+    d_hits = gpu.DeviceArray(n * gpu.HIT_DTYPE.itemsize)
+    try:
+        c_ref, p_ref = gpu.query_count(ctx, cam, d_rays, d_hits, n)
+        hits_ref = d_hits.download(np.empty(n, gpu.HIT_DTYPE))
+        c_exe, _ = gpu.query_count(ctx, cam, d_rays, d_hits, n, executed=True)
+        hits_exe = d_hits.download(np.empty(n, gpu.HIT_DTYPE))
+    finally:
+        d_rays.close()
+        d_hits.close()
+    print(f"[query] counted loads: device {c_ref}, executed {c_exe}, oracle {oc.as_dict()}", flush=True)
+    for k in ("grid", "meta", "macro", "node", "rope", "colinfo", "clip"):
+        assert c_ref[k] == oc.as_dict()[k], k
+        assert c_exe[k] <= c_ref[k], k
+    assert c_ref["tex_rd"] == 32 * n and c_ref["tex_wr"] == 32 * n
+    assert sum(p_ref.values()) == gpu.algorithmic_bytes(c_ref)
+    assert np.array_equal(hits_ref.view(np.uint8), ref.view(np.uint8))
+    assert np.array_equal(hits_exe.view(np.uint8), ref.view(np.uint8))
+    assert c_ref["node"] > 0 and c_ref["rope"] > 0
+
+
 def test_query_interface_edges(gpu):
     sc, hw = host_world("c0")
     ctx = make_context(gpu, hw)

@@ -237,6 +237,42 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_query(
   query_persistent(C, rays, hits, n, cursor, refill_below);
 }
 
+// Ray queries, counting forms: one ray per lane, traced alone (no refill) with
+// per-lane load counters, k_query_count_reference counting the reference
+// algorithm's loads and k_query_count_executed the loads k_query makes.  The
+// traversal's loads are charged to the trace part; the ray record read and the
+// hit record written count as texture bytes.
+#define VXD_COUNTING_QUERY(NS, NAME)                                                                                    \
+  __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void NAME(                                                    \
+      Scene S, const vx_camera* __restrict__ cam, const vx_ray* __restrict__ rays, vx_hit* __restrict__ hits,          \
+      uint32_t n, unsigned long long* __restrict__ counters) {                                                         \
+    NS::Ctx C;                                                                                                         \
+    C.S = S;                                                                                                           \
+    C.cam = cam;                                                                                                       \
+    C.grass_taper = nullptr;                                                                                           \
+    C.geom_only = true;                                                                                                \
+    Counts cnt = {};                                                                                                   \
+    cnt.part = kPartTrace;                                                                                             \
+    C.cnt = &cnt;                                                                                                      \
+    const uint32_t id = blockIdx.x * 64u + threadIdx.x;                                                                \
+    if (id < n) {                                                                                                      \
+      cnt.tex_rd += (uint32_t)sizeof(vx_ray);                                                                          \
+      cnt.part_bytes[kPartTextures] += (uint32_t)(sizeof(vx_ray) + sizeof(vx_hit));                                    \
+      const vx_ray r = rays[id];                                                                                       \
+      NS::QueryRay q;                                                                                                  \
+      NS::HitGeom h = NS::miss_hitgeom();                                                                              \
+      if (NS::query_begin(C, F3(r.ox, r.oy, r.oz), F3(r.dx, r.dy, r.dz), r.tmax, q))                                   \
+        while (!NS::query_step(C, q, h)) {                                                                             \
+        }                                                                                                              \
+      hits[id] = NS::query_result(h);                                                                                  \
+      cnt.tex_wr += (uint32_t)sizeof(vx_hit);                                                                          \
+    }                                                                                                                  \
+    flush_counts<true>(cnt, counters);                                                                                 \
+  }
+VXD_COUNTING_QUERY(counted, k_query_count_reference)
+VXD_COUNTING_QUERY(executed, k_query_count_executed)
+#undef VXD_COUNTING_QUERY
+
 // ---------------------------------------------------------------------------
 // Fused multi-view launches (a batch of views rendered together).
 //
@@ -599,5 +635,15 @@ hipError_t vx_launch_query(const vxd::Scene& S, const vx_camera* cam_dev, const 
                            uint32_t* cursor, uint32_t persistent, uint32_t refill_below, hipStream_t st) {
   hipLaunchKernelGGL(k_query_begin, dim3(1), dim3(64), 0, st, cursor);
   hipLaunchKernelGGL(k_query, dim3(persistent), dim3(64), 0, st, S, cam_dev, rays, hits, n, cursor, refill_below);
+  return hipGetLastError();
+}
+hipError_t vx_launch_query_count(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,
+                                 bool executed, unsigned long long* counters, hipStream_t st) {
+  const uint32_t groups = (n + 63u) / 64u;
+  if (!groups) return hipSuccess;
+  if (executed)
+    hipLaunchKernelGGL(k_query_count_executed, dim3(groups), dim3(64), 0, st, S, cam_dev, rays, hits, n, counters);
+  else
+    hipLaunchKernelGGL(k_query_count_reference, dim3(groups), dim3(64), 0, st, S, cam_dev, rays, hits, n, counters);
   return hipGetLastError();
 }

@@ -31,6 +31,11 @@ hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32
 // Ray queries over n rays (device arrays) as one persistent launch with lane refill.
 hipError_t vx_launch_query(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,
                            uint32_t* cursor, uint32_t persistent, uint32_t refill_below, hipStream_t st);
+// The ray queries' load-counting form: one ray per lane, the reference
+// algorithm's loads or (executed) those k_query makes, added into counters as
+// by vx_launch_primary_count; the hits are those of vx_launch_query.
+hipError_t vx_launch_query_count(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,
+                                 bool executed, unsigned long long* counters, hipStream_t st);
 // Per chunk slot: 1 when its node range holds a LIGHT leaf, else 0 (n_slots entries).
 hipError_t vx_launch_chunk_lights(const vxd::Scene& S, uint32_t* out, uint32_t n_slots, hipStream_t st);
 // Clipmap texels of n rectangles (device array, at most max_texels texels each)

@@ -97,6 +97,7 @@ struct vx_ctx {
   // Ray queries: the scene camera copy, the ray cursor, configuration, timing.
   vx_camera* qcam_dev = nullptr;
   uint32_t* qcursor = nullptr;
+  unsigned long long* qcounters = nullptr;  // vx_query_count's counters (kViewCounters)
   vx_query_config qcfg{8u, 48u, 0u, 0u};
   hipEvent_t qev[2] = {nullptr, nullptr};
   // Live objects created on this context (views, batches, servers, chunk
@@ -368,6 +369,7 @@ int vx_destroy(vx_ctx* ctx) {
   dfree(ctx->defer);
   dfree(ctx->qcam_dev);
   dfree(ctx->qcursor);
+  dfree(ctx->qcounters);
   for (auto& e : ctx->qev)
     if (e) hipEventDestroy(e);
   delete ctx;
@@ -1246,6 +1248,35 @@ int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_de
                            ctx->qcfg.refill_below, st));
   }
   VX_HIP(hipEventRecord(ctx->qev[1], st));
+  return VX_OK;
+}
+
+int vx_query_count(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n, vx_hit* hits_dev,
+                   int executed, uint64_t out[9], uint64_t parts[VX_COUNT_PARTS]) {
+  if (!ctx || !scene_cam || !out || (n && (!rays_dev || !hits_dev))) return vx_fail(VX_E_INVALID, "vx_query_count: null argument");
+  DeviceGuard g(ctx->device);
+  if (!ctx->qcam_dev) {
+    int rc = dalloc(&ctx->qcam_dev, 1);
+    if (!rc) rc = dalloc(&ctx->qcursor, 1);
+    for (int i = 0; i < 2 && !rc; ++i)
+      if (hipEventCreate(&ctx->qev[i]) != hipSuccess) rc = vx_fail(VX_E_DEVICE, "vx_query_count: hipEventCreate");
+    if (rc) return rc;
+  }
+  if (!ctx->qcounters) {
+    const int rc = dalloc(&ctx->qcounters, kViewCounters);
+    if (rc) return rc;
+  }
+  VX_HIP(hipStreamSynchronize(ctx->stream));
+  VX_HIP(hipMemset(ctx->qcounters, 0, kViewCounters * sizeof(unsigned long long)));
+  VX_HIP(hipMemcpy(ctx->qcam_dev, scene_cam, sizeof(vx_camera), hipMemcpyHostToDevice));
+  VX_HIP(vx_launch_query_count(make_scene(ctx, false), ctx->qcam_dev, rays_dev, hits_dev, n, executed != 0, ctx->qcounters,
+                               ctx->stream));
+  VX_HIP(hipStreamSynchronize(ctx->stream));
+  unsigned long long tmp[kViewCounters];
+  VX_HIP(hipMemcpy(tmp, ctx->qcounters, sizeof(tmp), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 9; ++i) out[i] = tmp[i];
+  if (parts)
+    for (int i = 0; i < VX_COUNT_PARTS; ++i) parts[i] = tmp[vxd::kNumCounts + i];
   return VX_OK;
 }
 

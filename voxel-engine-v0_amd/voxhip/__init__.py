@@ -219,6 +219,7 @@ def _load() -> C.CDLL:
         "vx_render_views_noalloc": ([vp, C.POINTER(vp), u32, u32, vp], C.c_int),
         "vx_query_rays": ([vp, C.POINTER(Camera), vp, u32, vp, vp], C.c_int),
         "vx_query_timing": ([vp, C.POINTER(f32)], C.c_int),
+        "vx_query_count": ([vp, C.POINTER(Camera), vp, u32, vp, C.c_int, C.POINTER(u64), C.POINTER(u64)], C.c_int),
         "vx_query_set_config": ([vp, C.POINTER(QueryConfig)], C.c_int),
         "vx_query_get_config": ([vp, C.POINTER(QueryConfig)], C.c_int),
         "vx_query_tune": ([vp, C.POINTER(Camera), vp, u32, vp, C.POINTER(QueryConfig)], C.c_int),
@@ -273,7 +274,7 @@ EXPORTED = (
     "vx_render", "vx_read_output", "vx_output_device_ptr", "vx_write_history", "vx_view_counters", "vx_view_timings", "vx_view_profile", "vx_render_views_noalloc", "vx_view_count_parts",
     "vx_server_create", "vx_server_destroy", "vx_server_submit", "vx_server_poll", "vx_server_edit",
     "vx_server_wait_edits", "vx_server_drain", "vx_server_close_stream", "vx_server_get_stats",
-    "vx_query_rays", "vx_query_timing", "vx_query_set_config", "vx_query_get_config", "vx_query_tune", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
+    "vx_query_rays", "vx_query_timing", "vx_query_count", "vx_query_set_config", "vx_query_get_config", "vx_query_tune", "vx_sync", "vx_batch_create", "vx_batch_destroy", "vx_batch_render", "vx_batch_timing",
     "vx_batch_set_config", "vx_batch_get_config", "vx_batch_tune", "vx_batch_set_rays", "vx_build_chunk", "vx_world_default", "vx_world_ground_height", "vx_world_fill_chunk",
     "vx_host_world_build", "vx_host_world_free", "vx_host_world_get", "vx_host_world_edit", "vx_camera_build",
     "vx_clipmap_build", "vx_clipmap_state_create", "vx_clipmap_state_destroy", "vx_clipmap_invalidate",
@@ -810,6 +811,18 @@ def query_rays(ctx: "Context", scene_cam: Camera, rays: np.ndarray) -> np.ndarra
     d_rays.close()
     d_hits.close()
     return out
+
+
+def query_count(ctx: "Context", scene_cam: Camera, rays: DeviceArray, hits: DeviceArray, n: int,
+                executed: bool = False) -> tuple[dict, dict]:
+    """vx_query_count: the load counts of a query over n device rays (the
+    reference algorithm's, or with executed those vx_query_rays makes), as
+    (counts by COUNT_NAMES, bytes by COUNT_PARTS).  Synchronous."""
+    out = (C.c_uint64 * len(COUNT_NAMES))()
+    parts = (C.c_uint64 * len(COUNT_PARTS))()
+    _check(LIB.vx_query_count(ctx._h, C.byref(scene_cam), rays.ptr, n, hits.ptr, 1 if executed else 0, out, parts),
+           "vx_query_count")
+    return dict(zip(COUNT_NAMES, (int(v) for v in out))), dict(zip(COUNT_PARTS, (int(v) for v in parts)))
 
 
 def query_timing(ctx: "Context") -> float:
