@@ -221,6 +221,12 @@ int vxd_cpu_primary(const cpu_scene* scene, const vx_camera* cam, uint32_t w, ui
   return 0;
 }
 
+// The device's floor of a quotient (floor_quot with quot_rcp), elementwise.
+int vxd_cpu_floor_quot(const float* x, const float* d, uint32_t n, float* out) {
+  for (uint32_t k = 0; k < n; ++k) out[k] = plain::floor_quot(x[k], d[k], plain::quot_rcp(d[k]));
+  return 0;
+}
+
 // The device float->int conversions, elementwise.
 int vxd_cpu_convert(const float* x, uint32_t n, int32_t* out_i, uint32_t* out_u) {
   for (uint32_t k = 0; k < n; ++k) {

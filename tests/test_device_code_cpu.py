@@ -114,6 +114,48 @@ def test_device_float_to_int_conversions():
     assert np.array_equal(got_u.astype(np.int64), want_u.astype(np.int64))
 
 
+def test_floor_quotient_equals_floor_of_the_division():
+    """The device's floor_quot(x, d) (a product with 1/d, checked against the
+    error bound, dividing only near integers) equals floorf(x / d) with the
+    correctly rounded division, bit for bit: on the voxel, macro-cell and chunk
+    sizes the passes divide by, on quotients that land next to integers (the
+    products k * d and their float neighbours), on random values, and on zero,
+    signed, subnormal, huge and non-finite operands."""
+    rng = np.random.default_rng(17)
+    divisors = [0.1, 0.05, 0.125, 0.2, 0.3, 1.0 / 3.0, 0.4, 3.2, 6.4, 12.8, 0.7, 1.7, 1e-3, 123.456]
+    divisors += list(np.ldexp(rng.uniform(1.0, 2.0, 8), rng.integers(-20, 20, 8)))
+    xs, ds = [], []
+    for d in np.array(divisors, np.float32):
+        k = np.arange(-300, 300, dtype=np.float32)
+        kd = (k * d).astype(np.float32)
+        near = [kd]
+        for steps in (1, 2, 3, 8):
+            near.append(np.nextafter(kd, np.float32(np.inf)))
+            near.append(np.nextafter(kd, np.float32(-np.inf)))
+            kd_up, kd_dn = near[-2], near[-1]
+            for _ in range(steps - 1):
+                kd_up = np.nextafter(kd_up, np.float32(np.inf))
+                kd_dn = np.nextafter(kd_dn, np.float32(-np.inf))
+            near += [kd_up, kd_dn]
+        rnd = [rng.uniform(-30.0 * d, 30.0 * d, 4000), rng.uniform(0.0, 8.0 * d, 4000),
+               np.ldexp(rng.uniform(-1.0, 1.0, 2000), rng.integers(-140, 127, 2000))]
+        x = np.concatenate(near + rnd).astype(np.float32)
+        xs.append(x)
+        ds.append(np.full(len(x), d, np.float32))
+    specials = np.array([0.0, -0.0, 1e-45, -1e-45, 1e-40, 3e38, -3e38, np.inf, -np.inf, np.nan, 1.0, -1.0, 0.5],
+                        np.float32)
+    for d in np.array([0.1, 1e-40, 1e-38, 3e38, 0.0, -0.0, np.inf, -np.inf, np.nan, -0.1, 1e-45], np.float32):
+        xs.append(np.concatenate([specials, rng.uniform(-10, 10, 500).astype(np.float32)]))
+        ds.append(np.full(len(xs[-1]), d, np.float32))
+    x, d = np.concatenate(xs), np.concatenate(ds)
+    with np.errstate(all="ignore"):
+        want = np.floor(x / d)
+    got = vxd_cpu.floor_quot(x, d)
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), (x[~same][:5], d[~same][:5], got[~same][:5], want[~same][:5])
+    assert len(x) > 400_000
+
+
 def test_device_transcendentals_match_oracle():
     """The device sin/cos/exp/log/pow (branch-free special cases, 32-bit
     quadrant) against the oracle's, bit for bit, on special values, the

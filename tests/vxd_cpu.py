@@ -50,6 +50,8 @@ def lib():
         _LIB.vxd_cpu_query.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_int]
         _LIB.vxd_cpu_query.restype = C.c_int
         _LIB.vxd_cpu_convert.argtypes = [vp, C.c_uint32, vp, vp]
+        _LIB.vxd_cpu_floor_quot.argtypes = [vp, vp, C.c_uint32, vp]
+        _LIB.vxd_cpu_floor_quot.restype = C.c_int
         _LIB.vxd_cpu_convert.restype = C.c_int
         _LIB.vxd_cpu_math.argtypes = [vp, vp, C.c_uint32, vp]
         _LIB.vxd_cpu_math.restype = C.c_int
@@ -83,6 +85,15 @@ def query_rays(world, cam, rays: np.ndarray, hits_dtype, threads: int = 0):
     assert lib().vxd_cpu_query(C.addressof(world.scene), C.addressof(cam), rays.ctypes.data, len(rays),
                                hits.ctypes.data, threads) == 0
     return hits
+
+
+def floor_quot(x: np.ndarray, d: np.ndarray) -> np.ndarray:
+    """The device code's floor of x / d (floor_quot), elementwise in float32."""
+    x = np.ascontiguousarray(x, np.float32)
+    d = np.ascontiguousarray(np.broadcast_to(np.asarray(d, np.float32), x.shape), np.float32)
+    out = np.zeros(len(x), np.float32)
+    assert lib().vxd_cpu_floor_quot(x.ctypes.data, d.ctypes.data, len(x), out.ctypes.data) == 0
+    return out
 
 
 def convert(x: np.ndarray):
