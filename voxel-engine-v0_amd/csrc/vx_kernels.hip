@@ -406,8 +406,12 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
       const bool prof = PROF && (V.cam->profile_flags & 1u) != 0u;
       C.prof = prof ? L.prof : nullptr;
       __syncthreads();  // the previous tile's lanes are done with L
+      // A view's tiles are taken bottom row first: the rows near the camera's
+      // feet (grass close up) cost the most and the sky rows the least, so a
+      // band's last tiles, and with them the launch's tail, are cheap ones.
+      // The order changes no result (each tile is traced on its own).
       if constexpr (SHAPE == 0u) {
-        const uint32_t tile = t - V.tile_base;
+        const uint32_t tile = V.tiles - 1u - (t - V.tile_base);
         const uint32_t wgx = tile % V.tiles_x, wgy = tile / V.tiles_x;
         tile_gather(C, L, wgx, wgy);
         // Every lane runs the pixel pass (those outside the image only help
@@ -415,8 +419,8 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
         const uint32_t gx = wgx * 8u + (threadIdx.x & 7u), gy = wgy * 8u + (threadIdx.x >> 3);
         plain::primary_pixel(C, plain::LdsList{L}, fb, gx, gy, gx < fb.rw && gy < fb.rh, plain::sky_bg(F3(0.0f, 1.0f, 0.0f)));
       } else {
-        const uint32_t xt = t - V.xt_base[SHAPE - 1u];
         const uint32_t xtx = V.xt_x[SHAPE - 1u];
+        const uint32_t xt = xtx * ((fb.rh + TH - 1u) / TH) - 1u - (t - V.xt_base[SHAPE - 1u]);
         const uint32_t gx = (xt % xtx) * TW + (threadIdx.x % TW), gy = (xt / xtx) * TH + (threadIdx.x / TW);
         // this pixel's 8x8 tile (clamped into the view for lanes past its edge)
         const uint32_t ty = V.tiles / V.tiles_x;
