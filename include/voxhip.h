@@ -332,7 +332,9 @@ int vx_query_timing(vx_ctx* ctx, float* ms);
  * loaded from [grid, meta, macro, node, rope, colinfo, clip], then bytes of
  * [ray records read, hit records written]; parts (may be NULL) as
  * vx_view_count_parts.  hits_dev receives the hits vx_query_rays writes.
- * Synchronous: waits for the context's stream before and after. */
+ * Synchronous: waits for the context's stream before and after.  It uses the
+ * context's query camera buffer, so it must not overlap a vx_query_rays
+ * issued on another stream. */
 int vx_query_count(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n,
                    vx_hit* hits_dev, int executed, uint64_t out[9], uint64_t parts[VX_COUNT_PARTS]);
 typedef struct vx_query_config {
