@@ -115,6 +115,8 @@ def test_query_interface_edges(gpu):
     d = gpu.DeviceArray(64)
     gpu.query_rays_device(ctx, cam, d, d, 0)  # n = 0: nothing to do
     ctx.sync()
+    counts, parts = gpu.query_count(ctx, cam, d, d, 0)  # n = 0: no loads
+    assert not any(counts.values()) and not any(parts.values())
     with pytest.raises(gpu.VoxhipError):
         gpu.query_config(ctx, waves_per_simd=0, refill_below=99)
     d.close()
