@@ -340,7 +340,10 @@ int vx_query_count(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_d
 typedef struct vx_query_config {
     uint32_t waves_per_simd;  /* persistent workgroups = CUs * 4 SIMDs * this (default 8) */
     uint32_t refill_below;    /* refill idle lanes when fewer lanes than this are busy (default 48) */
-    uint32_t pad[2];
+    uint32_t sort_rays;       /* 1 (default): a call of 32768 rays or more traces them sorted by direction,
+                                 then origin (a device radix sort), so a wavefront's rays walk the same
+                                 chunks; 0: input order.  The hits are the same either way. */
+    uint32_t pad;
 } vx_query_config;
 int vx_query_set_config(vx_ctx* ctx, const vx_query_config* cfg);
 int vx_query_get_config(const vx_ctx* ctx, vx_query_config* cfg);

@@ -36,15 +36,16 @@ def test_query_matches_oracle(gpu):
     rays[3, 3] = 1000.0
     clip, heights = worlds.empty_clipmap(gpu)
     ref = O.query_rays(O.World.from_host(hw, clip, heights), cam, rays, gpu.HIT_DTYPE)
-    for waves, refill in ((8, 48), (4, 1), (6, 64)):
-        gpu.query_config(ctx, waves_per_simd=waves, refill_below=refill)
+    # the 100 k rays are traced sorted by direction and origin (the default), or in input order
+    for waves, refill, sort in ((8, 48, True), (4, 1, False), (6, 64, True)):
+        gpu.query_config(ctx, waves_per_simd=waves, refill_below=refill, sort_rays=sort)
         got = gpu.query_rays(ctx, cam, rays)
         bad = np.nonzero(got.view(np.uint8).reshape(len(rays), -1).any(axis=1) !=
                          ref.view(np.uint8).reshape(len(rays), -1).any(axis=1))[0]
         same = np.array_equal(got.view(np.uint8), ref.view(np.uint8))
-        print(f"[query] waves={waves} refill={refill}: identical={same} hits={int(np.sum(got['kind'] == 1))} "
+        print(f"[query] waves={waves} refill={refill} sort={sort}: identical={same} hits={int(np.sum(got['kind'] == 1))} "
               f"ms={gpu.query_timing(ctx):.3f}", flush=True)
-        assert same, (waves, refill, bad[:5])
+        assert same, (waves, refill, sort, bad[:5])
     assert int(np.sum(ref["kind"] == 1)) > 10_000
 
 

@@ -42,7 +42,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", f"-I{REPO / 'include'}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["vx_kernels.hip", "vx_runtime.hip", "vx_server.hip", "vx_builder.hip"]
+HIP_SOURCES = ["vx_kernels.hip", "vx_runtime.hip", "vx_server.hip", "vx_builder.hip", "vx_sort.hip"]
 CXX_SOURCES = ["host_world.cpp"]
 
 

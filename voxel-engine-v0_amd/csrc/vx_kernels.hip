@@ -182,11 +182,12 @@ __global__ __launch_bounds__(64) void k_query_begin(uint32_t* __restrict__ curso
 }
 
 // The persistent lane-refill loop over rays [0, n) from `cursor` (shared by
-// the ray-query kernel and the fused batch launch).  Returns when the cursor
-// has passed n and no lane of the wavefront holds a ray.
+// the ray-query kernel and the fused batch launch), taken in the order `order`
+// gives (null: input order).  Returns when the cursor has passed n and no lane
+// of the wavefront holds a ray.
 __device__ __forceinline__ void query_persistent(const plain::Ctx& C, const vx_ray* __restrict__ rays,
                                                  vx_hit* __restrict__ hits, uint32_t n, uint32_t* __restrict__ cursor,
-                                                 uint32_t refill_below) {
+                                                 uint32_t refill_below, const uint32_t* __restrict__ order = nullptr) {
   plain::QueryRay q;
   bool active = false, drained = false;
   uint32_t id = 0u;
@@ -203,6 +204,7 @@ __device__ __forceinline__ void query_persistent(const plain::Ctx& C, const vx_r
         if (need) {
           id = base + (uint32_t)__popcll(m & below);
           if (id < n) {
+            if (order) id = order[id];
             const vx_ray r = rays[id];
             if (plain::query_begin(C, F3(r.ox, r.oy, r.oz), F3(r.dx, r.dy, r.dz), r.tmax, q)) active = true;
             else hits[id] = plain::query_result(plain::miss_hitgeom());
@@ -228,13 +230,44 @@ __device__ __forceinline__ void query_persistent(const plain::Ctx& C, const vx_r
 
 __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_query(
     Scene S, const vx_camera* __restrict__ cam, const vx_ray* __restrict__ rays, vx_hit* __restrict__ hits, uint32_t n,
-    uint32_t* __restrict__ cursor, uint32_t refill_below) {
+    uint32_t* __restrict__ cursor, uint32_t refill_below, const uint32_t* __restrict__ order) {
   plain::Ctx C;
   C.S = S;
   C.cam = cam;
   C.grass_taper = nullptr;
   C.geom_only = true;
-  query_persistent(C, rays, hits, n, cursor, refill_below);
+  query_persistent(C, rays, hits, n, cursor, refill_below, order);
+}
+
+// Sort keys of ray queries.  Incoherent rays traced in input order put rays
+// from all over the world, in all directions, into one wavefront; sorted by
+// this key, a wavefront's rays start in nearby space and point the same way,
+// so they walk the same chunks and macro cells and load the same nodes.  The
+// key only orders the work: each ray's result does not depend on it.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // bits b of v to bit 3b
+  uint32_t r = 0u;
+#pragma unroll
+  for (uint32_t b = 0; b < 5u; ++b) r |= ((v >> b) & 1u) << (3u * b);
+  return r;
+}
+__global__ __launch_bounds__(256) void k_query_keys(const vx_camera* __restrict__ cam, const vx_ray* __restrict__ rays,
+                                                    uint32_t n, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const vx_ray r = rays[i];
+  const float csm = (float)cam->chunk_size * cam->voxel_params[0];
+  const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
+  uint32_t qo[3], qd[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float lo = (float)cam->grid_origin_chunk[a] * csm, ext = (float)cam->grid_dims[a] * csm;
+    // saturating conversions: NaN and out-of-box values land in the end bins
+    qo[a] = (uint32_t)min(max(f2i((o[a] - lo) / ext * 32.0f), 0), 31);
+    qd[a] = (uint32_t)min(max(f2i((d[a] + 1.0f) * 4.0f), 0), 7);
+  }
+  keys[i] = ((spread3(qd[0]) | (spread3(qd[1]) << 1u) | (spread3(qd[2]) << 2u)) << 15u) |
+            spread3(qo[0]) | (spread3(qo[1]) << 1u) | (spread3(qo[2]) << 2u);
+  vals[i] = i;
 }
 
 // Ray queries, counting forms: one ray per lane, traced alone (no refill) with
@@ -638,7 +671,21 @@ hipError_t vx_launch_clip_fill(const vx_clip_rect* rects, uint32_t n, uint32_t m
 hipError_t vx_launch_query(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,
                            uint32_t* cursor, uint32_t persistent, uint32_t refill_below, hipStream_t st) {
   hipLaunchKernelGGL(k_query_begin, dim3(1), dim3(64), 0, st, cursor);
-  hipLaunchKernelGGL(k_query, dim3(persistent), dim3(64), 0, st, S, cam_dev, rays, hits, n, cursor, refill_below);
+  hipLaunchKernelGGL(k_query, dim3(persistent), dim3(64), 0, st, S, cam_dev, rays, hits, n, cursor, refill_below,
+                     (const uint32_t*)nullptr);
+  return hipGetLastError();
+}
+hipError_t vx_launch_query_ordered(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits,
+                                   uint32_t n, const uint32_t* order, uint32_t* cursor, uint32_t persistent,
+                                   uint32_t refill_below, hipStream_t st) {
+  hipLaunchKernelGGL(k_query_begin, dim3(1), dim3(64), 0, st, cursor);
+  hipLaunchKernelGGL(k_query, dim3(persistent), dim3(64), 0, st, S, cam_dev, rays, hits, n, cursor, refill_below, order);
+  return hipGetLastError();
+}
+hipError_t vx_launch_query_keys(const vx_camera* cam_dev, const vx_ray* rays, uint32_t n, uint32_t* keys, uint32_t* vals,
+                                hipStream_t st) {
+  static_assert(kQuerySortBits == 24u, "9 direction bits above 15 origin bits");
+  if (n) hipLaunchKernelGGL(k_query_keys, dim3((n + 255u) / 256u), dim3(256), 0, st, cam_dev, rays, n, keys, vals);
   return hipGetLastError();
 }
 hipError_t vx_launch_query_count(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,

@@ -31,6 +31,21 @@ hipError_t vx_launch_primary_ws(const vxd::Scene& S, const vxd::Batch& B, uint32
 // Ray queries over n rays (device arrays) as one persistent launch with lane refill.
 hipError_t vx_launch_query(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits, uint32_t n,
                            uint32_t* cursor, uint32_t persistent, uint32_t refill_below, hipStream_t st);
+// Ray queries in a given order: order[k] is the k-th ray to trace (nullptr:
+// input order).  Hits go to their rays' own positions.
+hipError_t vx_launch_query_ordered(const vxd::Scene& S, const vx_camera* cam_dev, const vx_ray* rays, vx_hit* hits,
+                                   uint32_t n, const uint32_t* order, uint32_t* cursor, uint32_t persistent,
+                                   uint32_t refill_below, hipStream_t st);
+// Sort keys of n rays (keys[i], vals[i] = i): direction bins (3 bits per axis)
+// above the origin's Morton code over the chunk grid's box (5 bits per axis),
+// kQuerySortBits bits in all.
+constexpr uint32_t kQuerySortBits = 24;
+hipError_t vx_launch_query_keys(const vx_camera* cam_dev, const vx_ray* rays, uint32_t n, uint32_t* keys, uint32_t* vals,
+                                hipStream_t st);
+// rocPRIM's radix sort of (key, value) pairs on bits [0, end_bit); with temp
+// null it only sets *temp_bytes.
+hipError_t vx_sort_pairs_u32(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                             const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, uint32_t end_bit, hipStream_t st);
 // The ray queries' load-counting form: one ray per lane, the reference
 // algorithm's loads or (executed) those k_query makes, added into counters as
 // by vx_launch_primary_count; the hits are those of vx_launch_query.

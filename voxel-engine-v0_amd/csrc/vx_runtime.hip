@@ -98,7 +98,14 @@ struct vx_ctx {
   vx_camera* qcam_dev = nullptr;
   uint32_t* qcursor = nullptr;
   unsigned long long* qcounters = nullptr;  // vx_query_count's counters (kViewCounters)
-  vx_query_config qcfg{8u, 48u, 0u, 0u};
+  // The sorted order of a call's rays: keys and indices, in and out (2 x qsort_cap
+  // each), and the radix sort's temporary storage.
+  uint32_t* qkeys = nullptr;
+  uint32_t* qvals = nullptr;
+  void* qsort_temp = nullptr;
+  size_t qsort_temp_bytes = 0;
+  uint32_t qsort_cap = 0;
+  vx_query_config qcfg{8u, 48u, 1u, 0u};
   hipEvent_t qev[2] = {nullptr, nullptr};
   // Live objects created on this context (views, batches, servers, chunk
   // builders).  vx_destroy refuses while any is alive: each of them launches
@@ -370,6 +377,9 @@ int vx_destroy(vx_ctx* ctx) {
   dfree(ctx->qcam_dev);
   dfree(ctx->qcursor);
   dfree(ctx->qcounters);
+  dfree(ctx->qkeys);
+  dfree(ctx->qvals);
+  if (ctx->qsort_temp) hipFree(ctx->qsort_temp);
   for (auto& e : ctx->qev)
     if (e) hipEventDestroy(e);
   delete ctx;
@@ -1227,6 +1237,10 @@ int vx_sync(vx_ctx* ctx) {
   return VX_OK;
 }
 
+// Calls with fewer rays trace them in input order: the sort would cost more
+// than the coherence it buys.
+constexpr uint32_t kQuerySortMin = 32768;
+
 int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n, vx_hit* hits_dev, void* stream) {
   if (!ctx || !scene_cam || (n && (!rays_dev || !hits_dev))) return vx_fail(VX_E_INVALID, "vx_query_rays: null argument");
   DeviceGuard g(ctx->device);
@@ -1244,8 +1258,35 @@ int vx_query_rays(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_de
     const vx_batch_config wave_cfg{ctx->qcfg.waves_per_simd, 1u, 0u, 0u};
     // never more persistent workgroups than 64-ray blocks
     const uint32_t groups = std::min(persistent_groups(wave_cfg), (n + 63u) / 64u);
-    VX_HIP(vx_launch_query(make_scene(ctx, false), ctx->qcam_dev, rays_dev, hits_dev, n, ctx->qcursor, groups,
-                           ctx->qcfg.refill_below, st));
+    const uint32_t* order = nullptr;
+    if (ctx->qcfg.sort_rays && n >= kQuerySortMin) {
+      // Trace in the order of the rays' sort keys (direction, then origin).
+      if (n > ctx->qsort_cap) {
+        VX_HIP(hipStreamSynchronize(st));  // a previous call may still read the old buffers
+        dfree(ctx->qkeys);
+        dfree(ctx->qvals);
+        if (ctx->qsort_temp) hipFree(ctx->qsort_temp);
+        ctx->qkeys = ctx->qvals = nullptr;
+        ctx->qsort_temp = nullptr;
+        ctx->qsort_cap = 0;
+        size_t bytes = 0;
+        VX_HIP(vx_sort_pairs_u32(nullptr, &bytes, nullptr, nullptr, nullptr, nullptr, n, kQuerySortBits, st));
+        int rc = dalloc(&ctx->qkeys, 2 * (size_t)n);
+        if (!rc) rc = dalloc(&ctx->qvals, 2 * (size_t)n);
+        if (!rc && hipMalloc(&ctx->qsort_temp, std::max<size_t>(bytes, 1)) != hipSuccess)
+          rc = vx_fail(VX_E_NOMEM, "vx_query_rays: sort storage of %zu bytes", bytes);
+        if (rc) return rc;
+        ctx->qsort_temp_bytes = std::max<size_t>(bytes, 1);
+        ctx->qsort_cap = n;
+      }
+      VX_HIP(vx_launch_query_keys(ctx->qcam_dev, rays_dev, n, ctx->qkeys, ctx->qvals, st));
+      size_t bytes = ctx->qsort_temp_bytes;
+      VX_HIP(vx_sort_pairs_u32(ctx->qsort_temp, &bytes, ctx->qkeys, ctx->qkeys + n, ctx->qvals, ctx->qvals + n, n,
+                               kQuerySortBits, st));
+      order = ctx->qvals + n;
+    }
+    VX_HIP(vx_launch_query_ordered(make_scene(ctx, false), ctx->qcam_dev, rays_dev, hits_dev, n, order, ctx->qcursor,
+                                   groups, ctx->qcfg.refill_below, st));
   }
   VX_HIP(hipEventRecord(ctx->qev[1], st));
   return VX_OK;
@@ -1291,8 +1332,9 @@ int vx_query_timing(vx_ctx* ctx, float* ms) {
 
 int vx_query_set_config(vx_ctx* ctx, const vx_query_config* cfg) {
   if (!ctx || !cfg) return vx_fail(VX_E_INVALID, "vx_query_set_config: null argument");
-  if (cfg->waves_per_simd < 1u || cfg->waves_per_simd > 16u || cfg->refill_below < 1u || cfg->refill_below > 64u)
-    return vx_fail(VX_E_INVALID, "vx_query_set_config: waves_per_simd in 1..16, refill_below in 1..64");
+  if (cfg->waves_per_simd < 1u || cfg->waves_per_simd > 16u || cfg->refill_below < 1u || cfg->refill_below > 64u ||
+      cfg->sort_rays > 1u)
+    return vx_fail(VX_E_INVALID, "vx_query_set_config: waves_per_simd in 1..16, refill_below in 1..64, sort_rays 0 or 1");
   ctx->qcfg = *cfg;
   return VX_OK;
 }

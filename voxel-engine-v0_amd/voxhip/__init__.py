@@ -148,10 +148,11 @@ TILE_SHAPES = ("8x8", "16x4", "32x2")  # vx_batch_config.tile_shape
 
 
 class QueryConfig(C.Structure):
-    _fields_ = [("waves_per_simd", C.c_uint32), ("refill_below", C.c_uint32), ("pad", C.c_uint32 * 2)]
+    _fields_ = [("waves_per_simd", C.c_uint32), ("refill_below", C.c_uint32), ("sort_rays", C.c_uint32),
+                ("pad", C.c_uint32)]
 
     def as_dict(self) -> dict:
-        return {"waves_per_simd": self.waves_per_simd, "refill_below": self.refill_below}
+        return {"waves_per_simd": self.waves_per_simd, "refill_below": self.refill_below, "sort_rays": self.sort_rays}
 
 
 class ServerConfig(C.Structure):
@@ -831,13 +832,16 @@ def query_timing(ctx: "Context") -> float:
     return float(ms.value)
 
 
-def query_config(ctx: "Context", waves_per_simd: int | None = None, refill_below: int | None = None) -> QueryConfig:
+def query_config(ctx: "Context", waves_per_simd: int | None = None, refill_below: int | None = None,
+                 sort_rays: bool | None = None) -> QueryConfig:
     """Reads (and, given values, sets) the context's ray-query configuration."""
     cfg = QueryConfig()
     _check(LIB.vx_query_get_config(ctx._h, C.byref(cfg)), "vx_query_get_config")
-    if waves_per_simd is not None or refill_below is not None:
+    if waves_per_simd is not None or refill_below is not None or sort_rays is not None:
         cfg.waves_per_simd = waves_per_simd or cfg.waves_per_simd
         cfg.refill_below = refill_below or cfg.refill_below
+        if sort_rays is not None:
+            cfg.sort_rays = 1 if sort_rays else 0
         _check(LIB.vx_query_set_config(ctx._h, C.byref(cfg)), "vx_query_set_config")
     return cfg
 
