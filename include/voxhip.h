@@ -340,7 +340,7 @@ int vx_query_count(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_d
 typedef struct vx_query_config {
     uint32_t waves_per_simd;  /* persistent workgroups = CUs * 4 SIMDs * this (default 8) */
     uint32_t refill_below;    /* refill idle lanes when fewer lanes than this are busy (default 48) */
-    uint32_t sort_rays;       /* 1 (default): a call of 32768 rays or more traces them sorted by direction,
+    uint32_t sort_rays;       /* 1 (default): a call of 2^20 rays or more traces them sorted by direction,
                                  then origin (a device radix sort), so a wavefront's rays walk the same
                                  chunks; 0: input order.  The hits are the same either way. */
     uint32_t pad;
@@ -348,7 +348,8 @@ typedef struct vx_query_config {
 int vx_query_set_config(vx_ctx* ctx, const vx_query_config* cfg);
 int vx_query_get_config(const vx_ctx* ctx, vx_query_config* cfg);
 /* Autotuner: runs the query for waves_per_simd in {4, 6, 8} x refill_below in
- * {16, 32, 48, 64}, keeps the fastest (returned in *chosen, may be NULL). */
+ * {16, 32, 48, 64}, keeps the fastest (returned in *chosen, may be NULL).
+ * sort_rays keeps its current value. */
 int vx_query_tune(vx_ctx* ctx, const vx_camera* scene_cam, const vx_ray* rays_dev, uint32_t n,
                   vx_hit* hits_dev, vx_query_config* chosen);
 

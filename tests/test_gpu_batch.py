@@ -106,7 +106,9 @@ def test_batch_with_fused_rays(gpu):
     qcam = gpu.camera_build(gpu.make_pose((1.0, 1.0, 1.0), 0.0, 0.0), (64, 64), (64, 64), 0, grid_origin=sc.grid_origin,
                             grid_dims=sc.grid_dims, chunk_count=hw.chunk_count)
     rng = np.random.default_rng(17)
-    ray_sets = [np.concatenate([_random_rays(rng, 150_000, sc.grid_dims), worlds.boundary_rays()]),
+    # the first set is large enough to be sorted before it is traced (2^20 rays or more), the second is not
+    n_random = 1_100_000
+    ray_sets = [np.concatenate([_random_rays(rng, n_random, sc.grid_dims), worlds.boundary_rays()]),
                 _random_rays(rng, 70_001, sc.grid_dims)]
     w = O.World.from_host(hw, clip, heights)
     for cfg in (dict(waves_per_simd=4, grab=1, use_graph=True), dict(waves_per_simd=6, grab=2, use_graph=False)):
@@ -140,7 +142,7 @@ def test_batch_with_fused_rays(gpu):
                 assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), (cfg, frame)
                 sample = np.sort(np.random.default_rng(frame).choice(n, 4000, replace=False))
                 if frame == 0:
-                    sample = np.union1d(sample, np.arange(150_000, n))  # the boundary rays
+                    sample = np.union1d(sample, np.arange(n_random, n))  # the boundary rays
                 oref = O.query_rays(w, qcam, rays[sample], gpu.HIT_DTYPE)
                 assert np.array_equal(got[sample].view(np.uint8), oref.view(np.uint8)), (cfg, frame)
                 hit_frac = float(np.mean(got["kind"] == 1))

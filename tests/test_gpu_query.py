@@ -36,7 +36,7 @@ def test_query_matches_oracle(gpu):
     rays[3, 3] = 1000.0
     clip, heights = worlds.empty_clipmap(gpu)
     ref = O.query_rays(O.World.from_host(hw, clip, heights), cam, rays, gpu.HIT_DTYPE)
-    # the 100 k rays are traced sorted by direction and origin (the default), or in input order
+    # sort_rays on (the default) or off: 100 k rays stay below the sort's 2^20 threshold either way
     for waves, refill, sort in ((8, 48, True), (4, 1, False), (6, 64, True)):
         gpu.query_config(ctx, waves_per_simd=waves, refill_below=refill, sort_rays=sort)
         got = gpu.query_rays(ctx, cam, rays)
@@ -47,6 +47,29 @@ def test_query_matches_oracle(gpu):
               f"ms={gpu.query_timing(ctx):.3f}", flush=True)
         assert same, (waves, refill, sort, bad[:5])
     assert int(np.sum(ref["kind"] == 1)) > 10_000
+
+
+def test_query_sorted_equals_input_order(gpu):
+    """A call of 2^20 rays or more is traced sorted by direction and origin:
+    its hits equal those of the same call in input order byte for byte, and
+    the oracle's on a sample."""
+    from oracle import oracle as O
+    from voxhip import scenes
+
+    sc, hw = host_world("c1")
+    ctx = make_context(gpu, hw)
+    cam = _scene_cam(gpu, sc, hw)
+    rays = scenes.query_rays(np.random.default_rng(23), 1_100_000, sc.grid_origin, sc.grid_dims, tmax_m=51.2)
+    got = {}
+    for sort in (True, False):
+        gpu.query_config(ctx, waves_per_simd=6, refill_below=16, sort_rays=sort)
+        got[sort] = gpu.query_rays(ctx, cam, rays)
+        print(f"[query] 1.1 M rays sort={sort}: {gpu.query_timing(ctx):.3f} ms", flush=True)
+    assert np.array_equal(got[True].view(np.uint8), got[False].view(np.uint8))
+    sample = np.sort(np.random.default_rng(1).choice(len(rays), 3000, replace=False))
+    clip, heights = worlds.empty_clipmap(gpu)
+    ref = O.query_rays(O.World.from_host(hw, clip, heights), cam, rays[sample], gpu.HIT_DTYPE)
+    assert np.array_equal(got[True][sample].view(np.uint8), ref.view(np.uint8))
 
 
 def test_query_boundary_rays_match_oracle(gpu):

@@ -483,7 +483,7 @@ __global__ __launch_bounds__(64) VXD_PRIMARY_OCCUPANCY void k_primary_batch(Scen
       C.cam = &R->cam;
       C.grass_taper = nullptr;
       C.geom_only = true;
-      query_persistent(C, R->rays, R->hits, n, &R->cursor, __builtin_amdgcn_readfirstlane(R->refill_below));
+      query_persistent(C, R->rays, R->hits, n, &R->cursor, __builtin_amdgcn_readfirstlane(R->refill_below), R->order);
     }
   }
 }

@@ -105,6 +105,7 @@ struct BatchRays {
   vx_camera cam;                            // the scene camera of the queries
   const vx_ray* rays;
   vx_hit* hits;
+  const uint32_t* order;                    // the rays in the order to trace them (null: input order)
   uint32_t n;
   uint32_t refill_below;                    // lanes refill when fewer than this are busy
   uint32_t cursor;                          // next ray
